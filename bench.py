@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched split-step Pade range march on one MI355X.
+
+Workload (default): the 10,000-ray Munk batch (nz = 5303 rows, np = 6 Pade
+terms, dr = 10 m, dz = 1 m, 100 Hz; cupyram_amd.workloads.munk_batch).  A
+bench "step" is one range step of the march over the whole batch.  The self-
+starter and ``--warmup`` steps run untimed; exactly ``--steps`` march steps are
+timed between device synchronisations, with inputs resident in HBM.
+
+value = sum over rays of nz x timed steps / wall time / 1e6 (Mcell-steps/s)
+for the complex64 march; the complex128 march of the same batch is reported
+beside it.  roofline.achieved is the algorithmic HBM traffic of the march
+kernel, np x (6 S + 3 R + S_k) bytes per cell-step (408 B for complex64 and
+np = 6), divided by the kernel's mean launch time measured with hipEvents on
+the plan's stream.  cpu_baseline times the FP64 CPU oracle (the PyRAM
+restatement) on a bounded sample of the same batch on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-steps/s (nz×range steps) c64 vs c128 at ≤0.5 dB TL err; HBM GB/s"
+
+
+def alg_bytes_per_cell_step(np_pade: int, precision: int) -> int:
+    s = 8 if precision == 64 else 16     # complex element
+    r = 4 if precision == 64 else 8      # real medium element
+    return np_pade * (6 * s + 3 * r + s)
+
+
+def time_march(model, steps, warmup, launch_steps):
+    """Selfs + warmup untimed, then `steps` timed march steps."""
+    import cupyram_amd  # noqa: F401
+    model.setup()
+    nc = model._total_batch
+    plan = model._make_plan(np.arange(nc))
+    try:
+        plan.selfs()
+        if warmup:
+            plan.march(0, warmup)
+        plan.sync()
+        ms0, n0 = plan.march_time()
+        t0 = time.perf_counter()
+        s = warmup
+        while s < warmup + steps:
+            e = min(s + launch_steps, warmup + steps)
+            plan.march(s, e)
+            s = e
+        plan.sync()
+        wall = time.perf_counter() - t0
+        ms1, n1 = plan.march_time()
+        variant = plan.describe(0)
+    finally:
+        plan.close()
+    cells = float(np.sum(model._rays["nz"].astype(np.float64)))
+    return {"wall_s": wall, "kernel_ms": ms1 - ms0, "launches": n1 - n0, "cells": cells,
+            "mcs": cells * steps / wall / 1e6, "variant": variant}
+
+
+def cpu_baseline(kw, sample_rays, sample_rmax, precision_runs):
+    """Oracle (FP64 PyRAM restatement, OpenMP over rays) on a seeded sample of
+    the batch with a shortened range; also the TL error of the GPU marches on
+    the same sample."""
+    from cupyram_amd import CuPyRAM
+    from oracle import ram_oracle as O
+    from tests import envs as E
+    rng = np.random.default_rng(7)
+    n = kw["batch_size"]
+    pick = np.sort(rng.choice(n, size=min(sample_rays, n), replace=False))
+    skw = dict(kw)
+    for k in ("cb", "rhob", "attn", "rbzb"):
+        if isinstance(kw[k], list):
+            skw[k] = [kw[k][i] for i in pick]
+    skw["batch_size"] = len(pick)
+    skw["rmax"] = sample_rmax
+    m = CuPyRAM(**skw)
+    m.setup()
+    rays = [E.oracle_for(m, e, 0) for e in range(len(pick))]
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    tl_ref = O.run_batch_tl(rays, nthreads=cores)
+    t = time.perf_counter() - t0
+    cell_steps = float(np.sum(m._rays["nz"].astype(np.float64) * m._rays["nsteps"]))
+    errs = {}
+    for prec in precision_runs:
+        g = CuPyRAM(**skw, precision=prec, variant=precision_runs[prec]).run()["TL Line"]
+        d = np.concatenate([g[i, :len(tl_ref[i])] - tl_ref[i] for i in range(len(pick))])
+        d = d[np.isfinite(d)]
+        errs[prec] = {"rms_db": float(np.sqrt(np.mean(d ** 2))), "max_db": float(np.max(np.abs(d))),
+                      "frac_gt_0.5db": float(np.mean(np.abs(d) > 0.5))}
+    return {"value": cell_steps / t / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{len(pick)} seeded rays of the batch, rmax {sample_rmax:.0f} m "
+                      f"({int(m._rays['nsteps'][0])} steps), full nz, FP64 oracle, "
+                      f"OpenMP over rays ({cell_steps:.3g} cell-steps in {t:.1f} s)"}, errs
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--rays", type=int, default=10000)
+    ap.add_argument("--variant", default="auto")
+    ap.add_argument("--launch-steps", type=int, default=100, help="range steps per kernel launch")
+    ap.add_argument("--no-c128", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-rays", type=int, default=16)
+    ap.add_argument("--cpu-rmax", type=float, default=1000.0)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(rays are independent: shard the ray list across processes)"}))
+        return 2
+    from cupyram_amd import CuPyRAM, workloads
+    kw = workloads.munk_batch(nrays=args.rays)
+    np_pade = kw["np"]
+    r64 = time_march(CuPyRAM(**kw, precision="c64", variant=args.variant), args.steps, args.warmup,
+                     args.launch_steps)
+    out_c128 = None
+    if not args.no_c128:
+        r128 = time_march(CuPyRAM(**kw, precision="c128", variant=args.variant), args.steps, args.warmup,
+                          args.launch_steps)
+        out_c128 = {"value": r128["mcs"], "ms_per_step": 1e3 * r128["wall_s"] / args.steps,
+                    "variant": r128["variant"],
+                    "kernel_ms_per_launch": r128["kernel_ms"] / max(r128["launches"], 1)}
+    per_launch_steps = args.steps / max(r64["launches"], 1)
+    bytes_per_launch = alg_bytes_per_cell_step(np_pade, 64) * r64["cells"] * per_launch_steps
+    avg_launch_s = r64["kernel_ms"] / max(r64["launches"], 1) / 1e3
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    line = {
+        "metric": METRIC, "value": r64["mcs"], "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * r64["wall_s"] / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (complex64 march; FP64 Pade and self-starter)",
+        "data": "synthetic (seeded Munk batch, random bathymetry and bottom)",
+        "config": {"workload": "munk_batch: 10,000-ray Munk 5000 m, 100 Hz, nz 5303, np 6, dr 10 m, dz 1 m",
+                   "rays": args.rays, "nz": int(r64["cells"] / args.rays), "np": np_pade,
+                   "variant": r64["variant"], "steps_per_launch": per_launch_steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_cell_step": alg_bytes_per_cell_step(np_pade, 64),
+                     "kernel_ms_per_launch": avg_launch_s * 1e3},
+        "c128": out_c128,
+    }
+    if out_c128:
+        line["speedup_c64_vs_c128"] = r64["mcs"] / out_c128["value"]
+    if not args.no_cpu:
+        cb, errs = cpu_baseline(kw, args.cpu_rays, args.cpu_rmax,
+                                {"c64": args.variant, "c128": args.variant})
+        line["cpu_baseline"] = cb
+        line["tl_err_vs_fp64_oracle"] = errs
+        line["speedup_vs_cpu"] = r64["mcs"] / cb["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

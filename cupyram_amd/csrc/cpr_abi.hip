@@ -1,0 +1,476 @@
+// C ABI of libcupyram_hip.so: plan lifetime, uploads, launches, read-back.
+// Declared in include/cupyram_hip.h; the reference interface each entry point
+// replaces is listed there.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "march_common.h"
+
+template <class R>
+__global__ void k_march_thread(MarchArgs a, int s0, int s1);
+template <class R>
+__global__ void k_field_from_starter(MarchArgs a, const cx<double> *ud);
+__global__ void k_selfs(MarchArgs a);
+int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, int s1, hipStream_t st);
+int team_field_in(const MarchArgs &a, int precision, hipStream_t st);
+bool team_shape_supported(int precision, int rows, int lanes);
+void team_auto_shape(int precision, int nz, int *rows, int *lanes);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &m) {
+  g_err = m;
+  return -1;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct cpr_plan {
+  cpr_config cfg{};
+  int nrays = 0, np_ = 0, nz_max = 0, max_steps = 0;
+  int64_t ld = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<cpr_ray> rays_h;
+  cpr_ray *rays_d = nullptr;
+  cpr_env *envs_d = nullptr;
+  double *env_d = nullptr;
+  double *pade_d = nullptr;
+  RayState *state_d = nullptr;
+  void *col[7] = {};  // f1 f2 f3 ksq u cp dp
+  double *tll_d = nullptr, *cpl_d = nullptr, *tlg_d = nullptr, *cpg_d = nullptr;
+  int variant = CPR_VARIANT_THREAD;
+  int team_rows = 0, team_lanes = 0;
+  std::vector<hipEvent_t> events;  // pairs around march launches
+  int64_t bytes = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+
+  MarchArgs args() const {
+    MarchArgs a{};
+    a.rays = rays_d;
+    a.envs = envs_d;
+    a.env = env_d;
+    a.pade = pade_d;
+    a.state = state_d;
+    a.nrays = nrays;
+    a.np_ = np_;
+    a.ld = ld;
+    a.nvr_max = cfg.nvr_max;
+    a.nvz_max = cfg.nvz_max;
+    a.grids = cfg.compute_grids;
+    a.f1 = col[0];
+    a.f2 = col[1];
+    a.f3 = col[2];
+    a.ksq = col[3];
+    a.u = col[4];
+    a.cp = col[5];
+    a.dp = col[6];
+    a.tll = tll_d;
+    a.cpl = cpl_d;
+    a.tlg = tlg_d;
+    a.cpg = cpg_d;
+    return a;
+  }
+  size_t real_size() const { return cfg.precision == CPR_C64 ? sizeof(float) : sizeof(double); }
+};
+
+namespace {
+
+template <class T>
+int dalloc(cpr_plan *p, T **ptr, size_t count) {
+  const size_t b = std::max<size_t>(count, 1) * sizeof(T);
+  HIPCHK(hipMalloc(reinterpret_cast<void **>(ptr), b));
+  HIPCHK(hipMemsetAsync(*ptr, 0, b, p->stream));
+  p->bytes += static_cast<int64_t>(b);
+  return 0;
+}
+
+int alloc_columns(cpr_plan *p, void **col, size_t rs) {
+  const size_t cells = static_cast<size_t>(p->nz_max + 2) * static_cast<size_t>(p->ld);
+  const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
+  for (int k = 0; k < 7; ++k) {
+    char *q = nullptr;
+    if (dalloc(p, &q, cells * sizes[k])) return -1;
+    col[k] = q;
+  }
+  return 0;
+}
+
+void free_columns(cpr_plan *p, void **col, size_t rs) {
+  const size_t cells = static_cast<size_t>(p->nz_max + 2) * static_cast<size_t>(p->ld);
+  const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
+  for (int k = 0; k < 7; ++k) {
+    if (col[k]) {
+      (void)hipFree(col[k]);
+      p->bytes -= static_cast<int64_t>(std::max<size_t>(cells * sizes[k], 1));
+      col[k] = nullptr;
+    }
+  }
+}
+
+int check_ray(const cpr_ray &r, const cpr_env &e, int64_t env_len, int b) {
+  char buf[256];
+  auto bad = [&](const char *what) {
+    std::snprintf(buf, sizeof(buf), "ray %d: %s", b, what);
+    return fail(buf);
+  };
+  if (r.nz < 3) return bad("nz < 3");
+  if (r.np < 1 || r.np > CPR_MAX_NP) return bad("np out of range");
+  if (!(r.dz > 0) || !(r.dr > 0) || !(r.zstep > 0)) return bad("dz, dr and zstep must be positive");
+  if (r.nsteps < 0 || r.ndr < 1 || r.ndz < 1) return bad("bad step counts");
+  if (r.iz0 < 1 || r.iz0 > r.nz - 1) return bad("iz0 outside 1..nz-1");
+  const int is = static_cast<int>(std::floor(r.zs / r.dz));
+  const int ir = static_cast<int>(std::floor(r.zr / r.dz));
+  if (is < 0 || is + 1 > r.nz + 1) return bad("source depth outside the grid");
+  if (ir < 0 || ir + 1 > r.nz + 1) return bad("receiver depth outside the grid");
+  if (r.nvz < 0 || static_cast<int64_t>(r.nvz) * r.ndz > r.nz + 1) return bad("output depths exceed the grid");
+  if (e.n_ss < 1 || e.n_rp_ss < 1 || e.n_sb < 1 || e.n_rp_sb < 1 || e.n_bt < 1) return bad("empty profile");
+  const int64_t ends[9] = {e.z_ss + e.n_ss,
+                           e.rp_ss + e.n_rp_ss,
+                           e.cw + static_cast<int64_t>(e.n_ss) * e.n_rp_ss,
+                           e.z_sb + e.n_sb,
+                           e.rp_sb + e.n_rp_sb,
+                           e.cb + static_cast<int64_t>(e.n_sb) * e.n_rp_sb,
+                           e.rhob + static_cast<int64_t>(e.n_sb) * e.n_rp_sb,
+                           e.attn + static_cast<int64_t>(e.n_sb) * e.n_rp_sb,
+                           e.rbzb + 2 * static_cast<int64_t>(e.n_bt)};
+  const int64_t starts[9] = {e.z_ss, e.rp_ss, e.cw, e.z_sb, e.rp_sb, e.cb, e.rhob, e.attn, e.rbzb};
+  for (int k = 0; k < 9; ++k)
+    if (starts[k] < 0 || ends[k] > env_len) return bad("environment offsets outside env_data");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cpr_last_error(void) { return g_err.c_str(); }
+
+int cpr_device_count(int *count) {
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  if (count) *count = n;
+  return 0;
+}
+
+int cpr_pade(double freq, double c0, int np_, int ns, double dr, int ip, double *pd1, double *pd2) {
+  const int rc = cpr_pade_impl(freq, c0, np_, ns, dr, ip, pd1, pd2);
+  if (rc == -1) return fail("Laguerre method not converging");
+  if (rc) return fail("invalid Pade arguments");
+  return 0;
+}
+
+int cpr_pade_batch(int n, const double *freq, const double *c0, int np_, int ns, const double *dr, int ip,
+                   double *out, int nthreads) {
+  const int rc = cpr_pade_batch_impl(n, freq, c0, np_, ns, dr, ip, out, nthreads);
+  if (rc == -1) return fail("Laguerre method not converging");
+  if (rc) return fail("invalid Pade arguments");
+  return 0;
+}
+
+int cpr_plan_destroy(cpr_plan *p) {
+  if (!p) return 0;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (auto e : p->events) (void)hipEventDestroy(e);
+  free_columns(p, p->col, p->real_size());
+  for (void *q : {static_cast<void *>(p->rays_d), static_cast<void *>(p->envs_d), static_cast<void *>(p->env_d),
+                  static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
+                  static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d)})
+    if (q) (void)hipFree(q);
+  if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return 0;
+}
+
+int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *envs, const double *env_data,
+                    int64_t env_len, const double *pade, void *stream, cpr_plan **out) {
+  if (!cfg || !rays || !envs || !env_data || !pade || !out) return fail("null argument");
+  *out = nullptr;
+  if (cfg->nrays < 1) return fail("nrays must be >= 1");
+  if (cfg->precision != CPR_C64 && cfg->precision != CPR_C128) return fail("precision must be 64 or 128");
+  if (cfg->nvr_max < 0 || cfg->nvz_max < 0) return fail("negative output sizes");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
+  const int np_ = rays[0].np;
+  int nz_max = 0, max_steps = 0;
+  for (int b = 0; b < cfg->nrays; ++b) {
+    if (check_ray(rays[b], envs[b], env_len, b)) return -1;
+    if (rays[b].np != np_) return fail("all rays of a plan must use the same np");
+    if (rays[b].nvr > cfg->nvr_max) return fail("ray nvr exceeds nvr_max");
+    if (cfg->compute_grids && rays[b].nvz > cfg->nvz_max) return fail("ray nvz exceeds nvz_max");
+    nz_max = std::max(nz_max, rays[b].nz);
+    max_steps = std::max(max_steps, rays[b].nsteps);
+  }
+  cpr_plan *p = new cpr_plan();
+  p->cfg = *cfg;
+  p->nrays = cfg->nrays;
+  p->np_ = np_;
+  p->nz_max = nz_max;
+  p->max_steps = max_steps;
+  p->ld = (static_cast<int64_t>(cfg->nrays) + 63) / 64 * 64;
+  p->rays_h.assign(rays, rays + cfg->nrays);
+  auto bail = [&](int rc) {
+    std::string m = g_err;
+    cpr_plan_destroy(p);
+    g_err = m;
+    return rc;
+  };
+  if (stream) {
+    p->stream = static_cast<hipStream_t>(stream);
+  } else {
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail("hipStreamCreate failed"));
+    p->own_stream = true;
+  }
+  // variant choice
+  p->variant = cfg->variant == CPR_VARIANT_AUTO ? CPR_VARIANT_THREAD : cfg->variant;
+  if (p->variant == CPR_VARIANT_TEAM) {
+    int rows = cfg->team_rows, lanes = cfg->team_lanes;
+    if (rows == 0 || lanes == 0) team_auto_shape(cfg->precision, nz_max, &rows, &lanes);
+    if (!team_shape_supported(cfg->precision, rows, lanes))
+      return bail(fail("unsupported team shape (rows per lane x lanes per ray)"));
+    if (static_cast<int64_t>(rows) * lanes < nz_max) return bail(fail("team shape too small for nz"));
+    p->team_rows = rows;
+    p->team_lanes = lanes;
+  } else if (p->variant != CPR_VARIANT_THREAD) {
+    return bail(fail("unknown variant"));
+  }
+  const int64_t npade = static_cast<int64_t>(cfg->nrays) * 3 * 2 * np_ * 2;
+  if (dalloc(p, &p->rays_d, cfg->nrays) || dalloc(p, &p->envs_d, cfg->nrays) || dalloc(p, &p->env_d, env_len) ||
+      dalloc(p, &p->pade_d, npade) || dalloc(p, &p->state_d, cfg->nrays))
+    return bail(-1);
+  if (alloc_columns(p, p->col, p->real_size())) return bail(-1);
+  const size_t nline = static_cast<size_t>(std::max(cfg->nvr_max, 1)) * p->ld;
+  if (dalloc(p, &p->tll_d, nline) || dalloc(p, &p->cpl_d, 2 * nline)) return bail(-1);
+  if (cfg->compute_grids) {
+    const size_t ngrid = nline * static_cast<size_t>(std::max(cfg->nvz_max, 1));
+    if (dalloc(p, &p->tlg_d, ngrid) || dalloc(p, &p->cpg_d, 2 * ngrid)) return bail(-1);
+  }
+  if (hipMemcpyAsync(p->rays_d, rays, sizeof(cpr_ray) * cfg->nrays, hipMemcpyHostToDevice, p->stream) ||
+      hipMemcpyAsync(p->envs_d, envs, sizeof(cpr_env) * cfg->nrays, hipMemcpyHostToDevice, p->stream) ||
+      hipMemcpyAsync(p->env_d, env_data, sizeof(double) * env_len, hipMemcpyHostToDevice, p->stream) ||
+      hipMemcpyAsync(p->pade_d, pade, sizeof(double) * npade, hipMemcpyHostToDevice, p->stream) ||
+      hipStreamSynchronize(p->stream))
+    return bail(fail("upload failed"));
+  *out = p;
+  return 0;
+}
+
+int cpr_selfs(cpr_plan *p) {
+  if (!p) return fail("null plan");
+  for (auto e : p->events) (void)hipEventDestroy(e);
+  p->events.clear();
+  MarchArgs a = p->args();
+  void *dcol[7] = {};
+  const bool own = p->cfg.precision == CPR_C64;
+  if (own) {
+    if (alloc_columns(p, dcol, sizeof(double))) return -1;
+    a.f1 = dcol[0];
+    a.f2 = dcol[1];
+    a.f3 = dcol[2];
+    a.ksq = dcol[3];
+    a.u = dcol[4];
+    a.cp = dcol[5];
+    a.dp = dcol[6];
+  }
+  const int block = 64;
+  const int grid = (p->nrays + block - 1) / block;
+  hipLaunchKernelGGL(k_selfs, dim3(grid), dim3(block), 0, p->stream, a);
+  HIPCHK(hipGetLastError());
+  if (own) {
+    MarchArgs m = p->args();
+    hipLaunchKernelGGL(k_field_from_starter<float>, dim3(grid), dim3(block), 0, p->stream, m,
+                       static_cast<const cx<double> *>(dcol[4]));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(p->stream));
+    free_columns(p, dcol, sizeof(double));
+  }
+  if (p->variant == CPR_VARIANT_TEAM && team_field_in(p->args(), p->cfg.precision, p->stream))
+    return -1;
+  return 0;
+}
+
+static int launch_march(cpr_plan *p, int s0, int s1) {
+  const MarchArgs a = p->args();
+  if (p->variant == CPR_VARIANT_TEAM)
+    return team_launch(a, p->cfg.precision, p->team_rows, p->team_lanes, s0, s1, p->stream);
+  const int block = 64;
+  const int grid = (p->nrays + block - 1) / block;
+  if (p->cfg.precision == CPR_C64)
+    hipLaunchKernelGGL(k_march_thread<float>, dim3(grid), dim3(block), 0, p->stream, a, s0, s1);
+  else
+    hipLaunchKernelGGL(k_march_thread<double>, dim3(grid), dim3(block), 0, p->stream, a, s0, s1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int cpr_march(cpr_plan *p, int step_begin, int step_end) {
+  if (!p) return fail("null plan");
+  if (step_begin < 0 || step_end < step_begin) return fail("bad step range");
+  step_end = std::min(step_end, p->max_steps);
+  if (step_end <= step_begin) return 0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  p->events.push_back(e0);
+  p->events.push_back(e1);
+  HIPCHK(hipEventRecord(e0, p->stream));
+  if (launch_march(p, step_begin, step_end)) return -1;
+  HIPCHK(hipEventRecord(e1, p->stream));
+  return 0;
+}
+
+int cpr_march_graph(cpr_plan *p, int steps_per_launch) {
+  if (!p) return fail("null plan");
+  if (steps_per_launch < 1) return fail("steps_per_launch must be >= 1");
+  if (p->graph_exec) {
+    (void)hipGraphExecDestroy(p->graph_exec);
+    p->graph_exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int s = 0; s < p->max_steps && !rc; s += steps_per_launch)
+    rc = launch_march(p, s, std::min(s + steps_per_launch, p->max_steps));
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(p->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  p->graph = g;
+  HIPCHK(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  p->events.push_back(e0);
+  p->events.push_back(e1);
+  HIPCHK(hipEventRecord(e0, p->stream));
+  HIPCHK(hipGraphLaunch(p->graph_exec, p->stream));
+  HIPCHK(hipEventRecord(e1, p->stream));
+  return 0;
+}
+
+int cpr_sync(cpr_plan *p) {
+  if (!p) return fail("null plan");
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int cpr_march_time(cpr_plan *p, double *ms, int *nlaunch) {
+  if (!p) return fail("null plan");
+  HIPCHK(hipStreamSynchronize(p->stream));
+  double tot = 0;
+  for (size_t i = 0; i + 1 < p->events.size(); i += 2) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, p->events[i], p->events[i + 1]));
+    tot += t;
+  }
+  if (ms) *ms = tot;
+  if (nlaunch) *nlaunch = static_cast<int>(p->events.size() / 2);
+  return 0;
+}
+
+int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double *cpg) {
+  if (!p) return fail("null plan");
+  HIPCHK(hipStreamSynchronize(p->stream));
+  const int64_t ld = p->ld, nvr = p->cfg.nvr_max, nvz = p->cfg.nvz_max, n = p->nrays;
+  std::vector<double> buf;
+  auto fetch = [&](const double *src, size_t count) -> int {
+    buf.resize(count);
+    HIPCHK(hipMemcpy(buf.data(), src, count * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  };
+  if (tll) {
+    if (fetch(p->tll_d, nvr * ld)) return -1;
+    for (int64_t b = 0; b < n; ++b)
+      for (int64_t t = 0; t < nvr; ++t) tll[b * nvr + t] = buf[t * ld + b];
+  }
+  if (cpl) {
+    if (fetch(p->cpl_d, 2 * nvr * ld)) return -1;
+    for (int64_t b = 0; b < n; ++b)
+      for (int64_t t = 0; t < nvr; ++t) {
+        cpl[2 * (b * nvr + t)] = buf[2 * (t * ld + b)];
+        cpl[2 * (b * nvr + t) + 1] = buf[2 * (t * ld + b) + 1];
+      }
+  }
+  if ((tlg || cpg) && !p->cfg.compute_grids) return fail("plan was created without grids");
+  if (tlg) {
+    if (fetch(p->tlg_d, nvr * nvz * ld)) return -1;
+    for (int64_t b = 0; b < n; ++b)
+      for (int64_t i = 0; i < nvz; ++i)
+        for (int64_t t = 0; t < nvr; ++t) tlg[(b * nvz + i) * nvr + t] = buf[(t * nvz + i) * ld + b];
+  }
+  if (cpg) {
+    if (fetch(p->cpg_d, 2 * nvr * nvz * ld)) return -1;
+    for (int64_t b = 0; b < n; ++b)
+      for (int64_t i = 0; i < nvz; ++i)
+        for (int64_t t = 0; t < nvr; ++t) {
+          cpg[2 * ((b * nvz + i) * nvr + t)] = buf[2 * ((t * nvz + i) * ld + b)];
+          cpg[2 * ((b * nvz + i) * nvr + t) + 1] = buf[2 * ((t * nvz + i) * ld + b) + 1];
+        }
+  }
+  return 0;
+}
+
+int cpr_read_field(cpr_plan *p, double *u, int nz_max) {
+  if (!p || !u) return fail("null argument");
+  if (nz_max < p->nz_max) return fail("nz_max smaller than the plan's");
+  HIPCHK(hipStreamSynchronize(p->stream));
+  const int64_t ld = p->ld, rows = p->nz_max + 2, n = p->nrays;
+  const bool c64 = p->cfg.precision == CPR_C64;
+  const size_t bytes = static_cast<size_t>(rows * ld) * (c64 ? 8 : 16);
+  std::vector<char> buf(bytes);
+  HIPCHK(hipMemcpy(buf.data(), p->col[4], bytes, hipMemcpyDeviceToHost));
+  std::memset(u, 0, sizeof(double) * 2 * n * (nz_max + 2));
+  for (int64_t b = 0; b < n; ++b)
+    for (int64_t i = 0; i < rows; ++i) {
+      double re, im;
+      if (c64) {
+        const float *f = reinterpret_cast<const float *>(buf.data());
+        re = f[2 * (i * ld + b)];
+        im = f[2 * (i * ld + b) + 1];
+      } else {
+        const double *d = reinterpret_cast<const double *>(buf.data());
+        re = d[2 * (i * ld + b)];
+        im = d[2 * (i * ld + b) + 1];
+      }
+      u[2 * (b * (nz_max + 2) + i)] = re;
+      u[2 * (b * (nz_max + 2) + i) + 1] = im;
+    }
+  return 0;
+}
+
+int64_t cpr_plan_device_bytes(cpr_plan *p) { return p ? p->bytes : -1; }
+
+int cpr_plan_describe(cpr_plan *p, int b, int *variant, int *team_rows, int *team_lanes) {
+  if (!p || b < 0 || b >= p->nrays) return fail("bad plan or ray");
+  if (variant) *variant = p->variant;
+  if (team_rows) *team_rows = p->team_rows;
+  if (team_lanes) *team_lanes = p->team_lanes;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,418 @@
+"""CuPyRAM host driver for the MI355X march.
+
+``CuPyRAM(freq, zs, zr, z_ss, rp_ss, cw, z_sb, rp_sb, cb, rhob, attn, rbzb,
+**kwargs).run()`` keeps the constructor, keyword defaults, grid rules and the
+returned dictionary of cupyram/cupyram.py:45-380.  The host resolves the
+per-ray parameters (cupyram.py:460-608), packs the environment tables, and
+computes the FP64 Pade coefficients; everything per range step -- range
+update, profiles, Galerkin assembly, the np tridiagonal solves and the TL
+output -- runs on the GPU inside libcupyram_hip.so with no host round trip.
+
+Deliberate differences from the reference, each a fix of a reference defect:
+  * the TL line is computed whether or not ``compute_grids`` is set (the
+    reference leaves it at zero with grids off, cupyram.py:306-307);
+  * the CP grid is written (outpt.py:50 assigns a local instead);
+  * the range-dependence of profiles and bathymetry is decided per ray, not
+    from ray 0 (cupyram.py:437-442);
+  * 'Proc Time' is wall-clock time of run() (the reference reports CPU
+    process time, cupyram.py:288,320);
+  * the self-starter's (1-X)^-1 smoothing applies one factor as PyRAM does
+    (the reference's fused kernel also sweeps np-1 identity factors).
+
+Extensions (keyword arguments the reference does not have):
+  precision='c128'|'c64'   arithmetic of the march (FP64 Pade + starter in both)
+  variant='auto'|'thread'|'team'   kernel family
+  freq_per_ray=True        ``freq`` is an array of one frequency per ray
+  dz / dr / rmax as arrays one value per ray: per-ray grids (then the
+                           wavelength of the absorbing layer is c0/f per ray)
+  steps_per_launch=k       march in launches of k range steps captured in a
+                           hipGraph (0: one launch for the whole march)
+"""
+from __future__ import annotations
+
+import time
+
+import numpy
+
+from . import _abi
+from .pade import compute_pade_batch
+
+
+class CuPyRAM:
+
+    _np_default = 8
+    _dzf = 0.1
+    _ndr_default = 1
+    _ndz_default = 1
+    _ns_default = 1
+    _lyrw_default = 20
+    _id_default = 0
+
+    # -- input normalisation (cupyram.py:55-174) --------------------------------
+
+    @staticmethod
+    def _normalize_to_batch(x, batch_size, param_name):
+        """List of per-ray arrays -> NaN-padded [batch, ...]; a single array or
+        scalar is tiled over the batch."""
+        if isinstance(x, list):
+            if len(x) != batch_size:
+                raise ValueError(f"{param_name}: list length {len(x)} != batch_size {batch_size}")
+            arrays = [numpy.asarray(item) for item in x]
+            shapes = {a.shape for a in arrays}
+            if len(shapes) == 1:
+                return numpy.stack(arrays, axis=0)
+            if len({a.ndim for a in arrays}) != 1:
+                raise ValueError(f"{param_name}: per-ray arrays differ in dimensionality")
+            ndim = arrays[0].ndim
+            full = tuple(max(a.shape[i] for a in arrays) for i in range(ndim))
+            dtype = arrays[0].dtype
+            if numpy.issubdtype(dtype, numpy.integer) or dtype == bool:
+                dtype = numpy.float64
+            out = numpy.full((batch_size,) + full, numpy.nan, dtype=dtype)
+            for i, a in enumerate(arrays):
+                out[(i,) + tuple(slice(0, s) for s in a.shape)] = a
+            return out
+        arr = numpy.asarray(x)
+        if arr.ndim == 0:
+            return numpy.full(batch_size, arr)
+        return numpy.tile(arr, (batch_size,) + (1,) * arr.ndim)
+
+    @staticmethod
+    def _get_valid_slice(arr):
+        """Strip NaN padding: 1-D up to the first NaN; 2-D drops all-NaN rows
+        and columns; returns (array, length)."""
+        arr = numpy.asarray(arr)
+        if arr.ndim == 1:
+            nan = numpy.isnan(arr)
+            if not nan.any():
+                return arr, len(arr)
+            n = int(numpy.argmax(nan))
+            return (arr[:n], n) if n > 0 else (arr, len(arr))
+        if arr.ndim == 2:
+            rows = int(numpy.sum(~numpy.all(numpy.isnan(arr), axis=1)))
+            cols = int(numpy.sum(~numpy.all(numpy.isnan(arr), axis=0)))
+            if rows == arr.shape[0] and cols == arr.shape[1]:
+                return arr, arr.shape[0]
+            return arr[:rows, :cols], rows
+        flat = arr.reshape(arr.shape[0], -1)
+        ok = ~numpy.any(numpy.isnan(flat), axis=1)
+        if ok.all():
+            return arr, len(arr)
+        n = int(numpy.argmax(~ok))
+        return (arr[:n], n) if n > 0 else (arr, len(arr))
+
+    # -- construction (cupyram.py:176-271) ---------------------------------------
+
+    def __init__(self, freq, zs, zr, z_ss, rp_ss, cw, z_sb, rp_sb, cb, rhob, attn, rbzb, **kwargs):
+        self._batch_size = int(kwargs.get("batch_size", 1))
+        self._compute_grids = bool(kwargs.get("compute_grids", True))
+        self._max_workers = kwargs.get("max_workers", 8)
+        self._precision = {"c64": 64, "complex64": 64, 64: 64, "c128": 128, "complex128": 128,
+                           128: 128}[kwargs.get("precision", "c128")]
+        variant = kwargs.get("variant", "auto")
+        if variant not in _abi.VARIANTS:
+            raise ValueError(f"variant must be one of {sorted(_abi.VARIANTS)}")
+        self._variant = variant
+        self._team_shape = kwargs.get("team_shape", None)
+        self._steps_per_launch = int(kwargs.get("steps_per_launch", 0))
+        self._freq_per_ray = bool(kwargs.get("freq_per_ray", False))
+        if numpy.isscalar(freq):
+            self._freqs = numpy.array([freq], dtype=numpy.float64)
+        else:
+            self._freqs = numpy.array(freq, dtype=numpy.float64).flatten()
+        if self._freq_per_ray:
+            if self._freqs.size == 1:
+                self._freqs = numpy.full(self._batch_size, self._freqs[0])
+            if self._freqs.size != self._batch_size:
+                raise ValueError("freq_per_ray needs one frequency per ray")
+            self._n_freq = 1
+        else:
+            self._n_freq = len(self._freqs)
+        if numpy.any(~(self._freqs > 0)):
+            raise ValueError("frequencies must be positive")
+        self._n_env = self._batch_size
+        self._total_batch = self._n_env * self._n_freq
+        self._zs, self._zr = float(zs), float(zr)
+        z_ss = self._normalize_to_batch(z_ss, self._batch_size, "z_ss")
+        rp_ss = self._normalize_to_batch(rp_ss, self._batch_size, "rp_ss")
+        cw = self._normalize_to_batch(cw, self._batch_size, "cw")
+        z_sb = self._normalize_to_batch(z_sb, self._batch_size, "z_sb")
+        rp_sb = self._normalize_to_batch(rp_sb, self._batch_size, "rp_sb")
+        cb = self._normalize_to_batch(cb, self._batch_size, "cb")
+        rhob = self._normalize_to_batch(rhob, self._batch_size, "rhob")
+        attn = self._normalize_to_batch(attn, self._batch_size, "attn")
+        rbzb = self._normalize_to_batch(rbzb, self._batch_size, "rbzb")
+        self.check_inputs(z_ss, rp_ss, cw, z_sb, rp_sb, cb, rhob, attn, rbzb)
+        self.get_params(**kwargs)
+
+    def check_inputs(self, z_ss, rp_ss, cw, z_sb, rp_sb, cb, rhob, attn, rbzb):
+        """Validate every ray and keep its valid (un-padded) tables
+        (cupyram.py:382-458)."""
+        self._status_ok = True
+        self._env = []
+        for b in range(self._batch_size):
+            zss, _ = self._get_valid_slice(numpy.asarray(z_ss[b], numpy.float64))
+            rss, _ = self._get_valid_slice(numpy.asarray(rp_ss[b], numpy.float64).ravel())
+            cwb = numpy.asarray(cw[b], numpy.float64)
+            if cwb.ndim == 1:
+                cwb = cwb[:, None]
+            cwb, _ = self._get_valid_slice(cwb)
+            zsb, _ = self._get_valid_slice(numpy.asarray(z_sb[b], numpy.float64).ravel())
+            rsb, _ = self._get_valid_slice(numpy.asarray(rp_sb[b], numpy.float64).ravel())
+            profs = []
+            for arr in (cb, rhob, attn):
+                a = numpy.asarray(arr[b], numpy.float64)
+                if a.ndim == 1:
+                    a = a[:, None]
+                profs.append(self._get_valid_slice(a)[0])
+            bt, _ = self._get_valid_slice(numpy.asarray(rbzb[b], numpy.float64).reshape(-1, 2))
+            if not (zss[0] <= self._zs <= zss[-1]):
+                raise ValueError(f"Ray {b}: Source depth {self._zs}m outside range [{zss[0]}, {zss[-1]}]m")
+            if not (zss[0] <= self._zr <= zss[-1]):
+                raise ValueError(f"Ray {b}: Receiver depth {self._zr}m outside range [{zss[0]}, {zss[-1]}]m")
+            if cwb.shape != (zss.size, rss.size):
+                raise ValueError(f"Ray {b}: z_ss ({zss.size}), rp_ss ({rss.size}), cw {cwb.shape} inconsistent")
+            for prof, name in zip(profs, ("cb", "rhob", "attn")):
+                if prof.shape != (zsb.size, rsb.size):
+                    raise ValueError(
+                        f"Ray {b}: z_sb ({zsb.size}), rp_sb ({rsb.size}), {name} {prof.shape} inconsistent")
+            if bt[:, 1].max() > zss[-1]:
+                raise ValueError(f"Ray {b}: Max bathy ({bt[:, 1].max()}m) > max SSP depth ({zss[-1]}m)")
+            if numpy.any(numpy.diff(zss) < 0) or numpy.any(numpy.diff(zsb) < 0):
+                raise ValueError(f"Ray {b}: profile depths must be non-decreasing")
+            self._env.append({"z_ss": zss, "rp_ss": rss, "cw": cwb, "z_sb": zsb, "rp_sb": rsb,
+                              "cb": profs[0], "rhob": profs[1], "attn": profs[2], "rbzb": bt})
+        e0 = self._env[0]
+        self.rd_ss = e0["rp_ss"].size > 1
+        self.rd_sb = e0["rp_sb"].size > 1
+        self.rd_bt = e0["rbzb"].shape[0] > 1
+
+    def get_params(self, **kwargs):
+        """Defaults of cupyram.py:460-526; per-ray arrays are an extension."""
+        self._np = int(kwargs.get("np", CuPyRAM._np_default))
+        if not 1 <= self._np <= _abi.MAX_NP:
+            raise ValueError(f"np must be in 1..{_abi.MAX_NP}")
+        self._benchmark_propagation = kwargs.get("benchmark_propagation", False)
+        if self._benchmark_propagation:
+            self.propagation_time = 0.0
+        if "c0" in kwargs:
+            self._c0 = kwargs["c0"]
+            self._c0_array = numpy.full(self._batch_size, float(kwargs["c0"]))
+        else:
+            self._c0_array = numpy.array([float(numpy.nanmean(e["cw"][:, 0])) for e in self._env])
+            self._c0 = float(numpy.mean(self._c0_array))
+        max_freq = float(numpy.max(self._freqs))
+        self._lambda = self._c0 / max_freq
+        self._per_ray_grid = any(numpy.ndim(kwargs.get(k, 0.0)) > 0 for k in ("dz", "dr", "rmax"))
+        self._dr = kwargs.get("dr", self._np * 1500 / max_freq)
+        self._dz = kwargs.get("dz", CuPyRAM._dzf * 1500 / max_freq)
+        self._ndr = int(kwargs.get("ndr", CuPyRAM._ndr_default))
+        self._ndz = int(kwargs.get("ndz", CuPyRAM._ndz_default))
+        self._zmplt = kwargs.get("zmplt", max(float(numpy.max(e["rbzb"][:, 1])) for e in self._env))
+        rmax_default = max(max(float(numpy.max(e["rp_ss"])), float(numpy.max(e["rp_sb"])),
+                               float(numpy.max(e["rbzb"][:, 0]))) for e in self._env)
+        self._rmax = kwargs.get("rmax", rmax_default)
+        self._rmax_input = self._rmax
+        self._ns = int(kwargs.get("ns", CuPyRAM._ns_default))
+        self._rs = kwargs.get("rs", numpy.asarray(self._rmax) + numpy.asarray(self._dr))
+        self._lyrw = kwargs.get("lyrw", CuPyRAM._lyrw_default)
+        self._id = kwargs.get("id", CuPyRAM._id_default)
+        self.proc_time = None
+
+    # -- setup (cupyram.py:528-726) ----------------------------------------------
+
+    def _per_env(self, value, n):
+        v = numpy.asarray(value, numpy.float64)
+        if v.ndim == 0:
+            return numpy.full(n, float(v))
+        if v.size != n:
+            raise ValueError("per-ray parameter arrays must have one value per ray")
+        return v.ravel()
+
+    def setup(self):
+        """Resolve per-ray grids, pack the environment and the Pade sets."""
+        ne, nf, nc = self._n_env, self._n_freq, self._total_batch
+        dz_e = self._per_env(self._dz, ne)
+        dr_e = self._per_env(self._dr, ne)
+        rmax_e = self._per_env(self._rmax_input, ne)
+        rs_e = self._per_env(self._rs, ne)
+        if self._freq_per_ray:
+            f_env = self._freqs
+        else:
+            f_env = numpy.full(ne, float(numpy.max(self._freqs)))
+        if self._per_ray_grid:
+            lam_e = self._c0_array / f_env
+        else:
+            lam_e = numpy.full(ne, self._lambda)
+        # seabed depths are relative to the last water-profile depth (cupyram.py:580-587)
+        zsb_abs = [e["z_sb"] + e["z_ss"][-1] for e in self._env]
+        zmax_sb = numpy.array([float(numpy.nanmax(z)) for z in zsb_abs])
+        if self._per_ray_grid:
+            zmax_e = zmax_sb + self._lyrw * lam_e
+        else:
+            zmax_e = numpy.full(ne, float(numpy.max(zmax_sb)) + self._lyrw * self._lambda)
+        nz_e = numpy.floor(zmax_e / dz_e).astype(numpy.int64) - 1
+        nzplt_e = numpy.floor(self._zmplt / dz_e).astype(numpy.int64)
+        nvr_e = numpy.floor(rmax_e / (dr_e * self._ndr)).astype(numpy.int64)
+        rmax_trunc = nvr_e * dr_e * self._ndr
+        nsteps_e = numpy.round(rmax_trunc / dr_e).astype(numpy.int64) - 1
+        nvz_e = numpy.floor(nzplt_e / self._ndz).astype(numpy.int64)
+        if numpy.any(nz_e < 3):
+            raise ValueError("depth grid too coarse (nz < 3)")
+        if numpy.any(nvz_e * self._ndz > nz_e + 1):
+            raise ValueError("zmplt lies below the computational grid")
+        # environment tables, extended bathymetry (cupyram.py:536-568)
+        chunks, envs, off = [], numpy.zeros(ne, _abi.ENV_DTYPE), 0
+        iz0_e = numpy.zeros(ne, numpy.int64)
+
+        def put(a):
+            nonlocal off
+            a = numpy.ascontiguousarray(a, numpy.float64).ravel()
+            chunks.append(a)
+            o = off
+            off += a.size
+            return o
+
+        for b, e in enumerate(self._env):
+            bt = e["rbzb"]
+            if bt[-1, 0] < rmax_e[b]:
+                bt = numpy.vstack([bt, [[rmax_e[b], bt[-1, 1]]]])
+            if bt.shape[0] == 1:
+                bt = numpy.vstack([bt, bt])
+                bt[1, 0] = bt[0, 0] + 1.0
+            iz0_e[b] = min(max(int(numpy.floor(bt[0, 1] / dz_e[b])), 1), nz_e[b] - 1)
+            rec = envs[b]
+            rec["n_ss"], rec["n_rp_ss"] = e["cw"].shape
+            rec["n_sb"], rec["n_rp_sb"] = e["cb"].shape
+            rec["n_bt"] = bt.shape[0]
+            rec["z_ss"] = put(e["z_ss"])
+            rec["rp_ss"] = put(e["rp_ss"])
+            rec["cw"] = put(e["cw"])
+            rec["z_sb"] = put(zsb_abs[b])
+            rec["rp_sb"] = put(e["rp_sb"])
+            rec["cb"] = put(e["cb"])
+            rec["rhob"] = put(e["rhob"])
+            rec["attn"] = put(e["attn"])
+            rec["rbzb"] = put(bt)
+        self._env_data = numpy.concatenate(chunks) if chunks else numpy.zeros(1)
+        env_of = numpy.arange(nc) // nf
+        if self._freq_per_ray:
+            f_calc = self._freqs
+        else:
+            f_calc = numpy.tile(self._freqs, ne)
+        rays = numpy.zeros(nc, _abi.RAY_DTYPE)
+        rays["dz"] = dz_e[env_of]
+        rays["zmax"] = zmax_e[env_of]
+        rays["nz"] = nz_e[env_of]
+        rays["zstep"] = zmax_e[env_of] / (nz_e[env_of] + 1)
+        rays["dr"] = dr_e[env_of]
+        rays["omega"] = 2 * numpy.pi * f_calc
+        rays["c0"] = self._c0_array[env_of]
+        rays["k0"] = rays["omega"] / rays["c0"]
+        rays["lyrw_lambda"] = self._lyrw * lam_e[env_of]
+        rays["rs"] = rs_e[env_of]
+        rays["zs"] = self._zs
+        rays["zr"] = self._zr
+        rays["nsteps"] = nsteps_e[env_of]
+        rays["iz0"] = iz0_e[env_of]
+        rays["ndr"] = self._ndr
+        rays["ndz"] = self._ndz
+        rays["nvr"] = nvr_e[env_of]
+        rays["nvz"] = nvz_e[env_of]
+        rays["np"] = self._np
+        self._rays = rays
+        self._envs = envs[env_of]
+        # Pade sets per calculation: march (ip=1, ns), starter (ip=2, ns), after rs (ip=1, ns=0)
+        pade = numpy.zeros((nc, 3, 2, self._np), numpy.complex128)
+        key = numpy.stack([f_calc, rays["c0"], rays["dr"]], axis=1)
+        uniq, inv = numpy.unique(key, axis=0, return_inverse=True)
+        inv = inv.ravel()
+        for s, (ip, ns) in enumerate(((1, self._ns), (2, self._ns), (1, 0))):
+            sets = compute_pade_batch(uniq[:, 0], uniq[:, 1], self._np, ns, uniq[:, 2], ip)
+            pade[:, s] = sets[inv]
+        self._pade = pade
+        # attributes of the reference after setup
+        self.nz = int(nz_e.max())
+        self.nzplt = int(nzplt_e.max())
+        self._zmax = float(zmax_e.max()) if not self._per_ray_grid else zmax_e
+        if not self._per_ray_grid:
+            self._rmax = float(rmax_trunc[0])
+        nvr, nvz = int(nvr_e.max()), int(nvz_e.max())
+        if self._per_ray_grid:
+            self.vr = [numpy.arange(1, n + 1) * d * self._ndr for n, d in zip(nvr_e, dr_e)]
+            self.vz = [numpy.arange(1, n + 1) * d * self._ndz for n, d in zip(nvz_e, dz_e)]
+        else:
+            self.vr = numpy.arange(1, nvr + 1) * dr_e[0] * self._ndr
+            self.vz = numpy.arange(1, nvz + 1) * dz_e[0] * self._ndz
+        self._nvr, self._nvz = nvr, nvz
+        self.eta = 1 / (40 * numpy.pi * numpy.log10(numpy.exp(1)))
+
+    # -- run (cupyram.py:273-380) ------------------------------------------------
+
+    def _make_plan(self, idx, stream=None):
+        cfg = _abi.CprConfig()
+        cfg.nrays = len(idx)
+        cfg.precision = self._precision
+        cfg.variant = _abi.VARIANTS[self._variant]
+        cfg.compute_grids = 1 if self._compute_grids else 0
+        cfg.nvr_max = self._nvr
+        cfg.nvz_max = self._nvz if self._compute_grids else 0
+        if self._team_shape:
+            cfg.team_rows, cfg.team_lanes = self._team_shape
+        return _abi.Plan(cfg, self._rays[idx], self._envs[idx], self._env_data, self._pade[idx], stream)
+
+    def run(self):
+        """March every ray; returns the reference's result dictionary."""
+        t0 = time.perf_counter()
+        self.setup()
+        nc = self._total_batch
+        plan = self._make_plan(numpy.arange(nc))
+        try:
+            plan.selfs()
+            nmax = int(self._rays["nsteps"].max())
+            if self._steps_per_launch > 0:
+                plan.march_graph(self._steps_per_launch)
+            else:
+                plan.march(0, nmax)
+            plan.sync()
+            self.march_ms, self.march_launches = plan.march_time()
+            tll, cpl, tlg, cpg = plan.read_outputs(grids=self._compute_grids)
+            self.plan_variant = plan.describe(0)
+        finally:
+            plan.close()
+        self.proc_time = time.perf_counter() - t0
+        if self._benchmark_propagation:
+            self.propagation_time = self.march_ms / 1e3
+        nvr_c = self._rays["nvr"]
+        if numpy.any(nvr_c < self._nvr):
+            for c in numpy.nonzero(nvr_c < self._nvr)[0]:
+                tll[c, nvr_c[c]:] = numpy.nan
+                cpl[c, nvr_c[c]:] = numpy.nan
+        self.tll, self.cpl, self.tlg, self.cpg = tll, cpl, tlg, cpg
+        ne, nf = self._n_env, self._n_freq
+        self.tll = self.tll.reshape(ne, nf, -1)
+        self.cpl = self.cpl.reshape(ne, nf, -1)
+        if self._compute_grids:
+            self.tlg = self.tlg.reshape(ne, nf, self.tlg.shape[1], self.tlg.shape[2])
+            self.cpg = self.cpg.reshape(ne, nf, self.cpg.shape[1], self.cpg.shape[2])
+        if nf == 1 and ne == 1:
+            self.tll, self.cpl = self.tll[0, 0], self.cpl[0, 0]
+            if self._compute_grids:
+                self.tlg, self.cpg = self.tlg[0, 0], self.cpg[0, 0]
+        elif nf == 1:
+            self.tll, self.cpl = self.tll[:, 0, :], self.cpl[:, 0, :]
+            if self._compute_grids:
+                self.tlg, self.cpg = self.tlg[:, 0], self.cpg[:, 0]
+        elif ne == 1:
+            self.tll, self.cpl = self.tll[0], self.cpl[0]
+            if self._compute_grids:
+                self.tlg, self.cpg = self.tlg[0], self.cpg[0]
+        return {"ID": self._id,
+                "Proc Time": self.proc_time,
+                "Ranges": self.vr,
+                "Depths": self.vz,
+                "TL Grid": self.tlg,
+                "TL Line": self.tll,
+                "CP Grid": self.cpg,
+                "CP Line": self.cpl,
+                "c0": self._c0}

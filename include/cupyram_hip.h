@@ -1,0 +1,123 @@
+/*
+ * cupyram_hip.h -- C ABI of the MI355X (gfx950) split-step Pade range march.
+ *
+ * Plain C types only: pointers, sizes, int status codes (0 = ok, <0 = error;
+ * cpr_last_error() returns the message of the last failure on this thread).
+ * No exception crosses the ABI.  The library owns all device memory of a
+ * plan; host pointers are borrowed for the duration of a call only.  One
+ * plan may be used by one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   cpr_pade / cpr_pade_batch   cupyram/pade.py:180       compute_pade_coefficients
+ *   cpr_plan_create             cupyram/cupyram.py:528    CuPyRAM.setup (device
+ *                               buffers, env upload) + cupyram/profl.py:142
+ *                               profl_cuda_launcher (profiles, now on device)
+ *   cpr_selfs                   cupyram/cupyram.py:1001   CuPyRAM.selfs
+ *   cpr_march                   cupyram/fused_kernel.py:157 fused_sum_pade_solve,
+ *                               cupyram/matrc.py:237 matrc_cuda_init_profiles,
+ *                               cupyram/updat.py:143 updat_indices_cuda,
+ *                               cupyram/outpt.py:61 outpt_cuda -- all fused,
+ *                               over a range of march steps, with no host sync
+ *   cpr_read_outputs            cupyram/cupyram.py:322-333 (D2H of tll/cpl/tlg/cpg)
+ */
+#ifndef CUPYRAM_HIP_H
+#define CUPYRAM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPR_MAX_NP 16
+
+/* precision of the march */
+enum { CPR_C64 = 64, CPR_C128 = 128 };
+/* kernel variant */
+enum { CPR_VARIANT_AUTO = 0, CPR_VARIANT_THREAD = 1, CPR_VARIANT_TEAM = 2 };
+
+/* Per-ray march parameters, resolved on the host the way CuPyRAM.get_params
+ * and setup do (cupyram/cupyram.py:460-608). */
+typedef struct {
+  double dz;          /* depth step used for indices (iz, ir, is) */
+  double zstep;       /* grid spacing zmax/(nz+1) of linspace(0, zmax, nz+2) */
+  double zmax;        /* last grid depth */
+  double dr;          /* range step */
+  double k0, omega, c0;
+  double lyrw_lambda; /* absorbing layer width (m) */
+  double rs;          /* range at which the stability constraints switch off */
+  double zs, zr;
+  int32_t nz;         /* interior rows 1..nz; the column has nz+2 rows */
+  int32_t nsteps;     /* march steps after the self-starter */
+  int32_t iz0;        /* initial bathymetry row */
+  int32_t ndr, ndz;
+  int32_t nvr, nvz;   /* output range / depth counts of this ray */
+  int32_t np;         /* Pade terms */
+} cpr_ray;
+
+/* Per-ray offsets (in doubles) into one packed environment array. */
+typedef struct {
+  int32_t n_ss, n_rp_ss;  /* water profile: depths x update ranges */
+  int32_t n_sb, n_rp_sb;  /* seabed profile */
+  int32_t n_bt;           /* bathymetry points (already extended to rmax) */
+  int32_t pad_;
+  int64_t z_ss, rp_ss, cw;          /* cw is [n_ss][n_rp_ss] */
+  int64_t z_sb, rp_sb, cb, rhob, attn; /* z_sb absolute (z_ss[-1] added) */
+  int64_t rbzb;                     /* [n_bt][2] */
+} cpr_env;
+
+typedef struct {
+  int32_t nrays;
+  int32_t precision;     /* CPR_C64 or CPR_C128 */
+  int32_t variant;       /* CPR_VARIANT_* */
+  int32_t compute_grids; /* 1: TL/CP grids every ndz rows */
+  int32_t nvr_max, nvz_max;
+  int32_t team_rows;     /* rows per lane of the team variant (0 = auto) */
+  int32_t team_lanes;    /* lanes per ray of the team variant (0 = auto) */
+} cpr_config;
+
+typedef struct cpr_plan cpr_plan;
+
+const char *cpr_last_error(void);
+int cpr_device_count(int *count);
+
+/* FP64 Pade coefficients; pd1/pd2 are complex128 as interleaved re,im pairs
+ * of length 2*np.  Returns -1 if Laguerre fails to converge. */
+int cpr_pade(double freq, double c0, int np_, int ns, double dr, int ip, double *pd1, double *pd2);
+/* n independent problems; out is [n][2][np] complex128 (pd1 then pd2). */
+int cpr_pade_batch(int n, const double *freq, const double *c0, int np_, int ns, const double *dr,
+                   int ip, double *out, int nthreads);
+
+/* Upload a batch.  pade is [nrays][3][2][np] complex128: set 0 = march
+ * (ip=1, ns), set 1 = self-starter (ip=2, ns), set 2 = march after rs
+ * (ip=1, ns=0).  stream is a hipStream_t (NULL: the plan makes its own). */
+int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *envs,
+                    const double *env_data, int64_t env_len, const double *pade, void *stream,
+                    cpr_plan **out);
+int cpr_plan_destroy(cpr_plan *p);
+/* Self-starter (FP64) and the first output; resets the march state. */
+int cpr_selfs(cpr_plan *p);
+/* March steps [step_begin, step_end) of every ray (steps past a ray's own
+ * nsteps are skipped).  Asynchronous on the plan's stream. */
+int cpr_march(cpr_plan *p, int step_begin, int step_end);
+/* Capture the launches of cpr_march(p, 0, nsteps) in chunks of
+ * steps_per_launch into a hipGraph and replay it. */
+int cpr_march_graph(cpr_plan *p, int steps_per_launch);
+int cpr_sync(cpr_plan *p);
+/* Time of the march kernels launched since the last cpr_selfs, measured with
+ * hipEvents on the plan's stream (ms), and the number of launches. */
+int cpr_march_time(cpr_plan *p, double *ms, int *nlaunch);
+/* Outputs, ray-major: tll [nrays][nvr_max] f64, cpl [nrays][nvr_max] c128,
+ * tlg [nrays][nvz_max][nvr_max] f64, cpg likewise c128.  Any may be NULL. */
+int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double *cpg);
+/* Field u after the last march step, [nrays][nz_max+2] complex128. */
+int cpr_read_field(cpr_plan *p, double *u, int nz_max);
+/* Bytes of device memory held by the plan. */
+int64_t cpr_plan_device_bytes(cpr_plan *p);
+/* Kernel variant and team shape chosen for ray b (for tests/autotuning). */
+int cpr_plan_describe(cpr_plan *p, int b, int *variant, int *team_rows, int *team_lanes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

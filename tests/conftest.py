@@ -1,0 +1,30 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+    config.addinivalue_line("markers", "slow: long-running")
+
+
+def gpu_available():
+    try:
+        from cupyram_amd import _abi
+        return _abi.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def require_gpu():
+    from cupyram_amd import _abi
+    n = _abi.device_count()
+    if n < 1:
+        pytest.fail("no HIP device visible; GPU tests must run on the GPU box")
+    return n
