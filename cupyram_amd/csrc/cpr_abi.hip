@@ -17,8 +17,8 @@ __global__ void k_march_thread(MarchArgs a, int s0, int s1);
 template <class R>
 __global__ void k_field_from_starter(MarchArgs a, const cx<double> *ud);
 __global__ void k_selfs(MarchArgs a);
+__global__ void k_tl_post(MarchArgs a);
 int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, int s1, hipStream_t st);
-int team_field_in(const MarchArgs &a, int precision, hipStream_t st);
 bool team_shape_supported(int precision, int rows, int lanes);
 void team_auto_shape(int precision, int nz, int *rows, int *lanes);
 
@@ -53,6 +53,7 @@ struct cpr_plan {
   RayState *state_d = nullptr;
   void *col[7] = {};  // f1 f2 f3 ksq u cp dp
   double *tll_d = nullptr, *cpl_d = nullptr, *tlg_d = nullptr, *cpg_d = nullptr;
+  int32_t *team_rays_d = nullptr;
   int variant = CPR_VARIANT_THREAD;
   int team_rows = 0, team_lanes = 0;
   std::vector<hipEvent_t> events;  // pairs around march launches
@@ -84,6 +85,8 @@ struct cpr_plan {
     a.cpl = cpl_d;
     a.tlg = tlg_d;
     a.cpg = cpg_d;
+    a.team_rays = team_rays_d;
+    a.team_nrays = nrays;
     return a;
   }
   size_t real_size() const { return cfg.precision == CPR_C64 ? sizeof(float) : sizeof(double); }
@@ -100,10 +103,11 @@ int dalloc(cpr_plan *p, T **ptr, size_t count) {
   return 0;
 }
 
-int alloc_columns(cpr_plan *p, void **col, size_t rs) {
+int alloc_columns(cpr_plan *p, void **col, size_t rs, bool only_u = false) {
   const size_t cells = static_cast<size_t>(p->nz_max + 2) * static_cast<size_t>(p->ld);
   const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
   for (int k = 0; k < 7; ++k) {
+    if (only_u && k != 4) continue;
     char *q = nullptr;
     if (dalloc(p, &q, cells * sizes[k])) return -1;
     col[k] = q;
@@ -193,7 +197,8 @@ int cpr_plan_destroy(cpr_plan *p) {
   free_columns(p, p->col, p->real_size());
   for (void *q : {static_cast<void *>(p->rays_d), static_cast<void *>(p->envs_d), static_cast<void *>(p->env_d),
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
-                  static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d)})
+                  static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d),
+                  static_cast<void *>(p->team_rays_d)})
     if (q) (void)hipFree(q);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -257,7 +262,14 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
   if (dalloc(p, &p->rays_d, cfg->nrays) || dalloc(p, &p->envs_d, cfg->nrays) || dalloc(p, &p->env_d, env_len) ||
       dalloc(p, &p->pade_d, npade) || dalloc(p, &p->state_d, cfg->nrays))
     return bail(-1);
-  if (alloc_columns(p, p->col, p->real_size())) return bail(-1);
+  if (alloc_columns(p, p->col, p->real_size(), p->variant == CPR_VARIANT_TEAM)) return bail(-1);
+  if (p->variant == CPR_VARIANT_TEAM) {
+    std::vector<int32_t> ids(cfg->nrays);
+    for (int b = 0; b < cfg->nrays; ++b) ids[b] = b;
+    if (dalloc(p, &p->team_rays_d, cfg->nrays)) return bail(-1);
+    if (hipMemcpy(p->team_rays_d, ids.data(), sizeof(int32_t) * cfg->nrays, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail("upload failed"));
+  }
   const size_t nline = static_cast<size_t>(std::max(cfg->nvr_max, 1)) * p->ld;
   if (dalloc(p, &p->tll_d, nline) || dalloc(p, &p->cpl_d, 2 * nline)) return bail(-1);
   if (cfg->compute_grids) {
@@ -280,7 +292,7 @@ int cpr_selfs(cpr_plan *p) {
   p->events.clear();
   MarchArgs a = p->args();
   void *dcol[7] = {};
-  const bool own = p->cfg.precision == CPR_C64;
+  const bool own = !(p->cfg.precision == CPR_C128 && p->variant == CPR_VARIANT_THREAD);
   if (own) {
     if (alloc_columns(p, dcol, sizeof(double))) return -1;
     a.f1 = dcol[0];
@@ -297,21 +309,26 @@ int cpr_selfs(cpr_plan *p) {
   HIPCHK(hipGetLastError());
   if (own) {
     MarchArgs m = p->args();
-    hipLaunchKernelGGL(k_field_from_starter<float>, dim3(grid), dim3(block), 0, p->stream, m,
-                       static_cast<const cx<double> *>(dcol[4]));
+    if (p->cfg.precision == CPR_C64)
+      hipLaunchKernelGGL(k_field_from_starter<float>, dim3(grid), dim3(block), 0, p->stream, m,
+                         static_cast<const cx<double> *>(dcol[4]));
+    else
+      hipLaunchKernelGGL(k_field_from_starter<double>, dim3(grid), dim3(block), 0, p->stream, m,
+                         static_cast<const cx<double> *>(dcol[4]));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(p->stream));
     free_columns(p, dcol, sizeof(double));
   }
-  if (p->variant == CPR_VARIANT_TEAM && team_field_in(p->args(), p->cfg.precision, p->stream))
-    return -1;
   return 0;
 }
 
 static int launch_march(cpr_plan *p, int s0, int s1) {
   const MarchArgs a = p->args();
-  if (p->variant == CPR_VARIANT_TEAM)
-    return team_launch(a, p->cfg.precision, p->team_rows, p->team_lanes, s0, s1, p->stream);
+  if (p->variant == CPR_VARIANT_TEAM) {
+    if (team_launch(a, p->cfg.precision, p->team_rows, p->team_lanes, s0, s1, p->stream))
+      return fail("team march launch failed");
+    return 0;
+  }
   const int block = 64;
   const int grid = (p->nrays + block - 1) / block;
   if (p->cfg.precision == CPR_C64)
@@ -395,6 +412,12 @@ int cpr_march_time(cpr_plan *p, double *ms, int *nlaunch) {
 
 int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double *cpg) {
   if (!p) return fail("null plan");
+  if (p->cfg.nvr_max > 0) {
+    const int block = 64;
+    hipLaunchKernelGGL(k_tl_post, dim3((p->nrays + block - 1) / block, p->cfg.nvr_max), dim3(block), 0, p->stream,
+                       p->args());
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipStreamSynchronize(p->stream));
   const int64_t ld = p->ld, nvr = p->cfg.nvr_max, nvz = p->cfg.nvz_max, n = p->nrays;
   std::vector<double> buf;

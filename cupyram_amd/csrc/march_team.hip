@@ -1,14 +1,498 @@
-// Team-of-lanes march (column resident in registers): placeholder entry
-// points until the kernel lands; every call reports that the variant is
-// unavailable.
+// Team-of-lanes march for gfx950: the whole column of a ray stays in
+// registers for every range step of a launch.
+//
+// A team of T lanes owns one ray; lane t holds rows g = 1 + t*M .. (t+1)*M
+// (M rows per lane, a template parameter, so every per-row array is fully
+// unrolled into VGPRs).  Rows past nz are identity rows (x = 0).  Per Pade
+// factor (M + p2 D) u' = (M + p1 D) u:
+//   1. each lane forms its RHS rows and eliminates downwards through its rows
+//      keeping the coupling to the previous lane's last unknown y_{t-1}
+//      ("spike" w): x_k + c'_k x_{k+1} + w_k y_{t-1} = d'_k;
+//   2. sweeps upwards to express its rows in y_{t-1} and its own last
+//      unknown y_t: x_k = D_k - W_k y_{t-1} - C_k y_t;
+//   3. its last row plus the next lane's first row give one tridiagonal
+//      equation in (y_{t-1}, y_t, y_{t+1}); the T equations are solved by
+//      parallel cyclic reduction across the team (wave shuffles when T <= 64,
+//      LDS otherwise);
+//   4. back-substitutes its rows.
+// This solves the same linear system as the reference's top-to-bottom
+// Thomas sweep (fused_kernel.py:124-152) in a different elimination order.
+// The range update (updat.py:38-140), medium rebuild where the water/bottom
+// split moved (matrc.py:63-107, profl.py:45-140) and the TL output
+// (outpt.py:11-58) are fused in the same persistent loop over range steps.
 #include <hip/hip_runtime.h>
 
 #include "march_common.h"
 
-int team_launch(const MarchArgs &, int, int, int, int, int, hipStream_t) { return -1; }
-int team_field_in(const MarchArgs &, int, hipStream_t) { return -1; }
-bool team_shape_supported(int, int, int) { return false; }
-void team_auto_shape(int, int, int *rows, int *lanes) {
-  *rows = 0;
-  *lanes = 0;
+namespace {
+
+template <class R>
+__device__ inline cx<R> fast_rcp(cx<R> a);
+template <>
+__device__ inline cx<float> fast_rcp(cx<float> a) {
+  const float m = __builtin_amdgcn_rcpf(a.re * a.re + a.im * a.im);
+  return cxm<float>(a.re * m, -a.im * m);
+}
+template <>
+__device__ inline cx<double> fast_rcp(cx<double> a) {
+  const double m = 1.0 / (a.re * a.re + a.im * a.im);
+  return cxm<double>(a.re * m, -a.im * m);
+}
+
+// Copy a team-uniform value into scalar registers.
+template <class T>
+__device__ inline T make_uniform(const T &v) {
+  static_assert(sizeof(T) % 4 == 0, "4-byte granularity");
+  T r;
+  const int *src = reinterpret_cast<const int *>(&v);
+  int *dst = reinterpret_cast<int *>(&r);
+#pragma unroll
+  for (int i = 0; i < static_cast<int>(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return r;
+}
+
+template <class R>
+__device__ inline R shfl_up_r(R v, int s, int w) {
+  return __shfl_up(v, s, w);
+}
+template <class R>
+__device__ inline R shfl_down_r(R v, int s, int w) {
+  return __shfl_down(v, s, w);
+}
+
+// Team exchange: values from lane t-s ("up") / t+s ("down"); zero outside
+// the team.  TW > 0: wave shuffles of width TW.  TW == 0: LDS with barriers.
+template <class R, int TW>
+struct Team {
+  int t, T;
+  cx<R> *lds;  // 3*T complex (LDS mode)
+
+  __device__ cx<R> up1(cx<R> v, int s) const {
+    if constexpr (TW > 0) {
+      cx<R> r = cxm<R>(shfl_up_r(v.re, s, TW), shfl_up_r(v.im, s, TW));
+      return t >= s ? r : cxm<R>(0, 0);
+    } else {
+      __syncthreads();
+      lds[t] = v;
+      __syncthreads();
+      return t >= s ? lds[t - s] : cxm<R>(0, 0);
+    }
+  }
+  __device__ cx<R> down1(cx<R> v, int s) const {
+    if constexpr (TW > 0) {
+      cx<R> r = cxm<R>(shfl_down_r(v.re, s, TW), shfl_down_r(v.im, s, TW));
+      return t + s < T ? r : cxm<R>(0, 0);
+    } else {
+      __syncthreads();
+      lds[t] = v;
+      __syncthreads();
+      return t + s < T ? lds[t + s] : cxm<R>(0, 0);
+    }
+  }
+  // halo: previous lane's `a` and next lane's `b`
+  __device__ void halo(cx<R> a, cx<R> b, cx<R> &from_prev, cx<R> &from_next) const {
+    if constexpr (TW > 0) {
+      from_prev = up1(a, 1);
+      from_next = down1(b, 1);
+    } else {
+      __syncthreads();
+      lds[t] = a;
+      lds[T + t] = b;
+      __syncthreads();
+      from_prev = t >= 1 ? lds[t - 1] : cxm<R>(0, 0);
+      from_next = t + 1 < T ? lds[T + t + 1] : cxm<R>(0, 0);
+    }
+  }
+  __device__ void next3(cx<R> a, cx<R> b, cx<R> c, cx<R> &na, cx<R> &nb, cx<R> &nc) const {
+    if constexpr (TW > 0) {
+      na = down1(a, 1);
+      nb = down1(b, 1);
+      nc = down1(c, 1);
+    } else {
+      __syncthreads();
+      lds[t] = a;
+      lds[T + t] = b;
+      lds[2 * T + t] = c;
+      __syncthreads();
+      const bool ok = t + 1 < T;
+      na = ok ? lds[t + 1] : cxm<R>(0, 0);
+      nb = ok ? lds[T + t + 1] : cxm<R>(0, 0);
+      nc = ok ? lds[2 * T + t + 1] : cxm<R>(0, 0);
+    }
+  }
+  // one PCR exchange: (a, c, f) of lanes t-s and t+s
+  __device__ void pcr(int s, cx<R> a, cx<R> c, cx<R> f, cx<R> &am, cx<R> &cm, cx<R> &fm, cx<R> &ap, cx<R> &cp,
+                      cx<R> &fp) const {
+    if constexpr (TW > 0) {
+      am = up1(a, s);
+      cm = up1(c, s);
+      fm = up1(f, s);
+      ap = down1(a, s);
+      cp = down1(c, s);
+      fp = down1(f, s);
+    } else {
+      __syncthreads();
+      lds[t] = a;
+      lds[T + t] = c;
+      lds[2 * T + t] = f;
+      __syncthreads();
+      const bool lo = t >= s, hi = t + s < T;
+      const cx<R> z = cxm<R>(0, 0);
+      am = lo ? lds[t - s] : z;
+      cm = lo ? lds[T + t - s] : z;
+      fm = lo ? lds[2 * T + t - s] : z;
+      ap = hi ? lds[t + s] : z;
+      cp = hi ? lds[T + t + s] : z;
+      fp = hi ? lds[2 * T + t + s] : z;
+    }
+  }
+};
+
+template <class R, int M>
+struct LaneCol {
+  cx<R> d1[M], d2[M], d3[M];  // Galerkin coefficients of the lane's rows
+  cx<R> u[M];                 // field, then d', then D
+  cx<R> cw[M];                // c', then C
+  cx<R> ww[M];                // w, then W
+};
+
+// Rebuild the Galerkin coefficients of the lane's rows from the medium.  The
+// FP64 medium of rows g0-1 .. g0+M is evaluated in a rolled loop with a
+// three-row window, the coefficients land in a small private array (this
+// runs only when the medium changed) and are then copied into the register
+// rows.  f3 of every row is kept for the TL output.
+template <class R, int M>
+__device__ void rebuild_rows(LaneCol<R, M> &L, R (&f3row)[M], const cpr_ray &P, const cpr_env &E,
+                             const double *env, const RayState &st, int g0) {
+  R dd[M][6], ff[M];
+  const double cfact = 0.5 / (P.dz * P.dz);
+  const double dfact = 1.0 / 12.0;
+  MediumRow w0, w1;
+#pragma unroll 1
+  for (int q = 0; q < M + 2; ++q) {
+    const int row = g0 - 1 + q;
+    MediumRow m;
+    if (row <= P.nz + 1) {
+      m = medium_at(P, E, env, st.ss_ind, st.sb_ind, st.iz, row);
+    } else {
+      m.f1 = m.f2 = m.f3 = 1.0;
+      m.kre = m.kim = 0.0;
+    }
+    if (q >= 2) {
+      const int k = q - 2;  // row g0+k uses the medium of rows g0+k-1 .. g0+k+1
+      const double c1 = cfact * w1.f1 * (w0.f2 + w1.f2) * w0.f3;
+      const double c2 = -cfact * w1.f1 * (w0.f2 + 2.0 * w1.f2 + m.f2) * w1.f3;
+      const double c3 = cfact * w1.f1 * (w1.f2 + m.f2) * m.f3;
+      dd[k][0] = static_cast<R>(c1 + dfact * (w0.kre + w1.kre));
+      dd[k][1] = static_cast<R>(dfact * (w0.kim + w1.kim));
+      dd[k][2] = static_cast<R>(c2 + dfact * (w0.kre + 6.0 * w1.kre + m.kre));
+      dd[k][3] = static_cast<R>(dfact * (w0.kim + 6.0 * w1.kim + m.kim));
+      dd[k][4] = static_cast<R>(c3 + dfact * (w1.kre + m.kre));
+      dd[k][5] = static_cast<R>(dfact * (w1.kim + m.kim));
+      ff[k] = static_cast<R>(w1.f3);
+    }
+    w0 = w1;
+    w1 = m;
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    L.d1[k] = cxm<R>(dd[k][0], dd[k][1]);
+    L.d2[k] = cxm<R>(dd[k][2], dd[k][3]);
+    L.d3[k] = cxm<R>(dd[k][4], dd[k][5]);
+    f3row[k] = ff[k];
+  }
+}
+
+template <class R, int M>
+__device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
+  cx<R> r = cxm<R>(0, 0);
+#pragma unroll
+  for (int q = 0; q < M; ++q)
+    if (q == k) r = v[q];
+  return r;
+}
+
+template <class R, int M>
+__device__ inline R pickr(const R (&v)[M], int k) {
+  R r = R(0);
+#pragma unroll
+  for (int q = 0; q < M; ++q)
+    if (q == k) r = v[q];
+  return r;
+}
+
+template <class R, int M, int TW>
+__device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *lds, int s0, int s1) {
+  const cpr_ray P = make_uniform(a.rays[b]);
+  const cpr_env E = make_uniform(a.envs[b]);
+  RayState st = make_uniform(a.state[b]);
+  Team<R, TW> team{t, T, lds};
+  const int nz = P.nz;
+  const int g0 = 1 + t * M;
+  int mv = nz - g0 + 1;  // valid rows of this lane
+  mv = mv < 0 ? 0 : (mv > M ? M : mv);
+  LaneCol<R, M> L;
+  R f3row[M];
+  cx<R> *ug = static_cast<cx<R> *>(a.u);
+#pragma unroll
+  for (int k = 0; k < M; ++k) L.u[k] = k < mv ? ug[static_cast<int64_t>(g0 + k) * a.ld + b] : cxm<R>(0, 0);
+  const R k0 = static_cast<R>(P.k0);
+  const R a1 = k0 * k0 / R(6), a2 = R(2) * k0 * k0 / R(3);
+  // receiver rows: owner lane / slot of rows ir and ir+1
+  const double ri = P.zr / P.dz;
+  const int ir = static_cast<int>(floor(ri));
+  const double dir = ri - ir;
+  bool fresh = true;
+  const int end = s1 < P.nsteps ? s1 : P.nsteps;
+  for (int step = s0; step < end; ++step) {
+    const double r = (step + 1) * P.dr;
+    const int jz = st.iz;
+    const int fl = __builtin_amdgcn_readfirstlane(updat_ray(P, E, a.env, st, r));
+    st = make_uniform(st);
+    // f changes on rows (lo, hi] when iz moves; d changes on rows lo .. hi+1
+    const int lo = jz < st.iz ? jz : st.iz, hi = jz > st.iz ? jz : st.iz;
+    const bool touched = (fl & 1) && g0 <= hi + 1 && g0 + M - 1 >= lo;
+    if (fresh || (fl & 2) || touched) rebuild_rows<R, M>(L, f3row, P, E, a.env, st, g0);
+    fresh = false;
+    const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
+    for (int j = 0; j < P.np; ++j) {
+      const cx<R> p1 = cxm<R>(static_cast<R>(pd[2 * j]), static_cast<R>(pd[2 * j + 1]));
+      const cx<R> p2 = cxm<R>(static_cast<R>(pd[2 * (P.np + j)]), static_cast<R>(pd[2 * (P.np + j) + 1]));
+      // 1. halo rows and downward elimination with the spike
+      cx<R> um, u_next;
+      team.halo(L.u[M - 1], L.u[0], um, u_next);
+      cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(-1, 0), dprev = cxm<R>(0, 0);
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const cx<R> uc = L.u[k];
+        const cx<R> up = (k + 1 < M) ? L.u[k + 1] : u_next;
+        const cx<R> r1 = cxm<R>(a1, 0) + p2 * L.d1[k];
+        const cx<R> r2 = cxm<R>(a2, 0) + p2 * L.d2[k];
+        const cx<R> r3 = cxm<R>(a1, 0) + p2 * L.d3[k];
+        const cx<R> du = L.d1[k] * um + L.d2[k] * uc + L.d3[k] * up;
+        const cx<R> rhs = a1 * (um + up) + a2 * uc + p1 * du;
+        const cx<R> inv = fast_rcp(cfms(r2, r1, cprev));
+        cx<R> cc = r3 * inv;
+        cx<R> wk = cxm<R>(0, 0) - (r1 * wprev) * inv;
+        cx<R> dk = cfms(rhs, r1, dprev) * inv;
+        if (k >= mv) {
+          cc = wk = dk = cxm<R>(0, 0);
+        }
+        L.cw[k] = cc;
+        L.ww[k] = wk;
+        L.u[k] = dk;
+        cprev = cc;
+        wprev = wk;
+        dprev = dk;
+        um = uc;
+      }
+      // 2. upward sweep: x_k = D_k - W_k y_{t-1} - C_k y_t for k < M-1
+#pragma unroll
+      for (int k = M - 3; k >= 0; --k) {
+        L.u[k] = cfms(L.u[k], L.cw[k], L.u[k + 1]);
+        L.ww[k] = cfms(L.ww[k], L.cw[k], L.ww[k + 1]);
+        L.cw[k] = cxm<R>(0, 0) - L.cw[k] * L.cw[k + 1];
+      }
+      // 3. reduced equation a y_{t-1} + b y_t + c y_{t+1} = f, then PCR
+      cx<R> nD, nW, nC;
+      team.next3(L.u[0], L.ww[0], L.cw[0], nD, nW, nC);
+      const cx<R> ce = L.cw[M - 1];
+      cx<R> ea = t > 0 ? L.ww[M - 1] : cxm<R>(0, 0);
+      cx<R> eb = cxm<R>(1, 0) - ce * nW;
+      cx<R> ec = cxm<R>(0, 0) - ce * nC;
+      cx<R> ef = cfms(L.u[M - 1], ce, nD);
+      {
+        const cx<R> ib = fast_rcp(eb);
+        ea = ea * ib;
+        ec = ec * ib;
+        ef = ef * ib;
+      }
+      for (int s = 1; s < T; s <<= 1) {
+        cx<R> am, cm, fm, ap, cp, fp;
+        team.pcr(s, ea, ec, ef, am, cm, fm, ap, cp, fp);
+        const cx<R> bn = cxm<R>(1, 0) - ea * cm - ec * ap;
+        const cx<R> an = cxm<R>(0, 0) - ea * am;
+        const cx<R> cn = cxm<R>(0, 0) - ec * cp;
+        const cx<R> fn = ef - ea * fm - ec * fp;
+        const cx<R> ib = fast_rcp(bn);
+        ea = an * ib;
+        ec = cn * ib;
+        ef = fn * ib;
+      }
+      // 4. back substitution
+      const cx<R> yc = ef;
+      const cx<R> yl = team.up1(yc, 1);
+#pragma unroll
+      for (int k = 0; k < M - 1; ++k) {
+        const cx<R> x = L.u[k] - L.ww[k] * yl - L.cw[k] * yc;
+        L.u[k] = k < mv ? x : cxm<R>(0, 0);
+      }
+      L.u[M - 1] = (M - 1) < mv ? yc : cxm<R>(0, 0);
+    }
+    // output (outpt.py:11-58): complex pressure only; TL is formed by a
+    // post-pass over the whole output array
+    st.mdr += 1;
+    if (st.mdr == P.ndr) {
+      st.mdr = 0;
+      st.tlc += 1;
+      if (st.tlc < P.nvr) {
+        const int ta = ir >= 1 ? (ir - 1) / M : 0, ka = ir >= 1 ? (ir - 1) % M : 0;
+        const int tb = ir / M, kb = ir % M;
+        cx<R> va = pick<R, M>(L.u, ka), vb = pick<R, M>(L.u, kb);
+        R fa = pickr<R, M>(f3row, ka), fb = pickr<R, M>(f3row, kb);
+        cx<R> ua, ub;
+        R f3a, f3b;
+        if constexpr (TW > 0) {
+          ua = cxm<R>(__shfl(va.re, ta, TW), __shfl(va.im, ta, TW));
+          f3a = __shfl(fa, ta, TW);
+          const int tbs = tb < T ? tb : 0;
+          ub = cxm<R>(__shfl(vb.re, tbs, TW), __shfl(vb.im, tbs, TW));
+          f3b = __shfl(fb, tbs, TW);
+        } else {
+          __syncthreads();
+          if (t == ta) {
+            lds[0] = va;
+            lds[2] = cxm<R>(fa, 0);
+          }
+          if (t == tb) {
+            lds[1] = vb;
+            lds[3] = cxm<R>(fb, 0);
+          }
+          __syncthreads();
+          ua = lds[0];
+          ub = lds[1];
+          f3a = lds[2].re;
+          f3b = lds[3].re;
+        }
+        if (ir < 1) {
+          ua = cxm<R>(0, 0);
+          f3a = R(0);
+        }
+        if (ir + 1 > nz) {
+          ub = cxm<R>(0, 0);
+          f3b = R(0);
+        }
+        if (t == 0) {
+          const double pre = (1.0 - dir) * static_cast<double>(f3a) * static_cast<double>(ua.re) +
+                             dir * static_cast<double>(f3b) * static_cast<double>(ub.re);
+          const double pim = (1.0 - dir) * static_cast<double>(f3a) * static_cast<double>(ua.im) +
+                             dir * static_cast<double>(f3b) * static_cast<double>(ub.im);
+          const int64_t o = static_cast<int64_t>(st.tlc) * a.ld + b;
+          a.cpl[2 * o] = pre;
+          a.cpl[2 * o + 1] = pim;
+        }
+        if (a.grids) {
+#pragma unroll
+          for (int k = 0; k < M; ++k) {
+            const int g = g0 + k;
+            if (k < mv && g % P.ndz == 0 && g / P.ndz <= P.nvz) {
+              const double f3 = static_cast<double>(f3row[k]);
+              const int64_t og = (static_cast<int64_t>(st.tlc) * a.nvz_max + (g / P.ndz - 1)) * a.ld + b;
+              a.cpg[2 * og] = static_cast<double>(L.u[k].re) * f3;
+              a.cpg[2 * og + 1] = static_cast<double>(L.u[k].im) * f3;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+    if (k < mv) ug[static_cast<int64_t>(g0 + k) * a.ld + b] = L.u[k];
+  if (t == 0) a.state[b] = st;
+}
+
+}  // namespace
+
+// Shuffle mode: one 64-lane team per wavefront, four per 256-thread block.
+template <class R, int M>
+__global__ void __launch_bounds__(256) k_march_team_w(MarchArgs a, int s0, int s1) {
+  const int team = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
+  if (team >= a.team_nrays) return;
+  const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
+  march_team_body<R, M, 64>(a, b, threadIdx.x % 64, 64, nullptr, s0, s1);
+}
+
+// LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64).
+template <class R, int M>
+__global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s1) {
+  extern __shared__ char smem[];
+  const int team = blockIdx.x;
+  if (team >= a.team_nrays) return;
+  const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
+  march_team_body<R, M, 0>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<cx<R> *>(smem), s0, s1);
+}
+
+namespace {
+
+template <class R, int M>
+int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
+  if (lanes == 64) {
+    const int grid = (a.team_nrays + 3) / 4;
+    hipLaunchKernelGGL((k_march_team_w<R, M>), dim3(grid), dim3(256), 0, st, a, s0, s1);
+  } else {
+    if (lanes % 64 || lanes > 512) return -1;
+    const size_t lds = 3 * static_cast<size_t>(lanes) * sizeof(cx<R>);
+    hipLaunchKernelGGL((k_march_team_l<R, M>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace
+
+bool team_shape_supported(int precision, int rows, int lanes) {
+  const bool lanes_ok = lanes >= 64 && lanes <= 512 && lanes % 64 == 0;
+  if (!lanes_ok) return false;
+  if (precision == CPR_C64) return rows == 4 || rows == 8 || rows == 12 || rows == 16;
+  return rows == 2 || rows == 4 || rows == 6 || rows == 8;
+}
+
+void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
+  const int cand64[] = {16, 12, 8, 4};
+  const int cand128[] = {8, 6, 4, 2};
+  const int *cand = precision == CPR_C64 ? cand64 : cand128;
+  int best_r = 0, best_l = 0;
+  long best_cost = -1;
+  for (int i = 0; i < 4; ++i) {
+    const int m = cand[i];
+    int l = (nz + m - 1) / m;
+    l = l <= 64 ? 64 : (l + 63) / 64 * 64;
+    if (l > 512) continue;
+    // padded rows plus a reduced-system term per lane
+    const long cost = static_cast<long>(l) * m + 8L * l;
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best_r = m;
+      best_l = l;
+    }
+  }
+  *rows = best_r;
+  *lanes = best_l;
+}
+
+int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, int s1, hipStream_t st) {
+  if (precision == CPR_C64) {
+    switch (rows) {
+      case 4:
+        return launch_rows<float, 4>(a, lanes, s0, s1, st);
+      case 8:
+        return launch_rows<float, 8>(a, lanes, s0, s1, st);
+      case 12:
+        return launch_rows<float, 12>(a, lanes, s0, s1, st);
+      case 16:
+        return launch_rows<float, 16>(a, lanes, s0, s1, st);
+    }
+  } else {
+    switch (rows) {
+      case 2:
+        return launch_rows<double, 2>(a, lanes, s0, s1, st);
+      case 4:
+        return launch_rows<double, 4>(a, lanes, s0, s1, st);
+      case 6:
+        return launch_rows<double, 6>(a, lanes, s0, s1, st);
+      case 8:
+        return launch_rows<double, 8>(a, lanes, s0, s1, st);
+    }
+  }
+  return -1;
 }

@@ -102,8 +102,10 @@ __device__ void factor_solve(const Col<R> &c, int nz, R k0, R cfact, cx<R> p1, c
   }
 }
 
+// Complex pressure at the receiver and on the output grid (outpt.py:39-53);
+// the TL of every stored pressure is formed afterwards by k_tl_post.
 template <class R>
-__device__ void outpt(const MarchArgs &a, const cpr_ray &P, RayState &st, const Col<R> &c, double r) {
+__device__ void outpt(const MarchArgs &a, const cpr_ray &P, RayState &st, const Col<R> &c) {
   st.mdr += 1;
   if (st.mdr != P.ndr) return;
   st.mdr = 0;
@@ -117,7 +119,6 @@ __device__ void outpt(const MarchArgs &a, const cpr_ray &P, RayState &st, const 
   const double pre = (1.0 - dir) * f30 * static_cast<double>(u0.re) + dir * f31 * static_cast<double>(u1.re);
   const double pim = (1.0 - dir) * f30 * static_cast<double>(u0.im) + dir * f31 * static_cast<double>(u1.im);
   const int64_t o = static_cast<int64_t>(st.tlc) * a.ld + c.b;
-  a.tll[o] = tl_of(pre, pim, r);
   a.cpl[2 * o] = pre;
   a.cpl[2 * o + 1] = pim;
   if (a.grids) {
@@ -127,7 +128,6 @@ __device__ void outpt(const MarchArgs &a, const cpr_ray &P, RayState &st, const 
       const double f3 = static_cast<double>(c.F3(j));
       const double gre = static_cast<double>(uj.re) * f3, gim = static_cast<double>(uj.im) * f3;
       const int64_t og = (static_cast<int64_t>(st.tlc) * a.nvz_max + i) * a.ld + c.b;
-      a.tlg[og] = tl_of(gre, gim, r);
       a.cpg[2 * og] = gre;
       a.cpg[2 * og + 1] = gim;
     }
@@ -171,7 +171,7 @@ __global__ void k_selfs(MarchArgs a) {
   const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + 1) * 2 * P.np * 2;
   for (int j = 0; j < P.np; ++j)
     factor_solve<double>(c, P.nz, P.k0, cfact, pade_coef<double>(pd, j), pade_coef<double>(pd, P.np + j));
-  outpt<double>(a, P, st, c, P.dr);
+  outpt<double>(a, P, st, c);
   a.state[b] = st;
 }
 
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(64) k_march_thread(MarchArgs a, int s0, int s1
     const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
     for (int j = 0; j < P.np; ++j)
       factor_solve<R>(c, P.nz, k0, cfact, pade_coef<R>(pd, j), pade_coef<R>(pd, P.np + j));
-    outpt<R>(a, P, st, c, (step + 2) * P.dr);
+    outpt<R>(a, P, st, c);
   }
   a.state[b] = st;
 }
@@ -223,3 +223,22 @@ template __global__ void k_march_thread<float>(MarchArgs, int, int);
 template __global__ void k_march_thread<double>(MarchArgs, int, int);
 template __global__ void k_field_from_starter<float>(MarchArgs, const cx<double> *);
 template __global__ void k_field_from_starter<double>(MarchArgs, const cx<double> *);
+
+// TL = -20 log10(|p| + 1e-20) + 10 log10(r + 1e-20) of every stored output
+// (outpt.py:43-53); output t of a ray lies at r = (t+1) ndr dr.
+__global__ void k_tl_post(MarchArgs a) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.y;
+  if (b >= a.nrays || t >= a.nvr_max) return;
+  const cpr_ray &P = a.rays[b];
+  if (t >= P.nvr) return;
+  const double r = (t + 1) * P.ndr * P.dr;
+  const int64_t o = static_cast<int64_t>(t) * a.ld + b;
+  a.tll[o] = tl_of(a.cpl[2 * o], a.cpl[2 * o + 1], r);
+  if (a.grids) {
+    for (int i = 0; i < P.nvz; ++i) {
+      const int64_t og = (static_cast<int64_t>(t) * a.nvz_max + i) * a.ld + b;
+      a.tlg[og] = tl_of(a.cpg[2 * og], a.cpg[2 * og + 1], r);
+    }
+  }
+}

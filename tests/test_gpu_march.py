@@ -17,7 +17,7 @@ from tests import envs
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "tl_ref.line")
-VARIANTS = ["thread"]
+VARIANTS = ["thread", "team"]
 
 
 def _run(kw, **extra):
