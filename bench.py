@@ -114,8 +114,8 @@ def main(argv=None):
     ap.add_argument("--launch-steps", type=int, default=100, help="range steps per kernel launch")
     ap.add_argument("--no-c128", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rays", type=int, default=16)
-    ap.add_argument("--cpu-rmax", type=float, default=1000.0)
+    ap.add_argument("--cpu-rays", type=int, default=128)
+    ap.add_argument("--cpu-rmax", type=float, default=5000.0)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "

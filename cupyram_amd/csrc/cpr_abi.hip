@@ -246,7 +246,13 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     p->own_stream = true;
   }
   // variant choice
-  p->variant = cfg->variant == CPR_VARIANT_AUTO ? CPR_VARIANT_THREAD : cfg->variant;
+  p->variant = cfg->variant;
+  if (p->variant == CPR_VARIANT_AUTO) {
+    // the register-resident team march when a team shape holds the column
+    int rows = 0, lanes = 0;
+    team_auto_shape(cfg->precision, nz_max, &rows, &lanes);
+    p->variant = (rows > 0 && static_cast<int64_t>(rows) * lanes >= nz_max) ? CPR_VARIANT_TEAM : CPR_VARIANT_THREAD;
+  }
   if (p->variant == CPR_VARIANT_TEAM) {
     int rows = cfg->team_rows, lanes = cfg->team_lanes;
     if (rows == 0 || lanes == 0) team_auto_shape(cfg->precision, nz_max, &rows, &lanes);

@@ -1,0 +1,278 @@
+"""Pins of the CPU oracle (oracle/ram_oracle.c), independent of the GPU.
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+  * RAM v1.5 golden TL line (tests/test_reference.py:20-28): mean |dTL| <= 0.01 dB;
+  * Pade: the rational function prod (1 + pd1 X)/(1 + pd2 X) matches the
+    Taylor series of g(X) = exp(i s (sqrt(1+X)-1)) (1+X)^a (1-X)^(2 nu),
+    computed here by series composition (not by the oracle's Leibniz
+    recursion), to order 2np (ns=0) or 2np-1 (ns=1); the numerator vanishes at
+    X=-3 (ns>=1) and X=-1.5 (ns=2);
+  * profiles: numpy.interp with clamping, and the absorbing-layer ramp in
+    closed form;
+  * bathymetry rows: brute force over the steps;
+  * elimination: identities of the factor solve (pd1 == pd2 is the identity
+    map; a factor followed by its swapped factor returns the input;
+    linearity);
+  * Pekeris waveguide: a textbook normal-mode sum (Jensen et al., Computational
+    Ocean Acoustics, ch. 5) with the lossy bottom and the bottom half-space in
+    the mode normalisation.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ram_oracle as O
+from tests import envs
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "tl_ref.line")
+
+
+def test_ram_v15_golden():
+    ref = np.loadtxt(GOLDEN)
+    out = O.run(O.resolve_ray(**envs.ram_reference_case()))
+    assert np.array_equal(out["Ranges"], ref[:, 0])
+    d = np.abs(out["TL Line"] - ref[:, 1])
+    assert d.mean() <= 0.01, d.mean()
+    assert d.max() <= 0.05, d.max()
+
+
+# -- Pade ----------------------------------------------------------------------
+
+def _series_mul(a, b, n):
+    return np.convolve(a, b)[:n]
+
+
+def _series_exp(h, n):
+    """exp of a power series with h[0] = 0 (recurrence e' = h' e)."""
+    e = np.zeros(n, complex)
+    e[0] = 1.0
+    dh = np.array([(k + 1) * h[k + 1] for k in range(n - 1)])
+    for k in range(1, n):
+        e[k] = np.sum(dh[:k] * e[k - 1::-1][:k]) / k
+    return e
+
+
+def _g_series(sig, ip, n):
+    """Taylor coefficients of g(X) from binomial / log series."""
+    k = np.arange(n)
+    # sqrt(1+X) - 1 = sum_{k>=1} binom(1/2, k) X^k
+    b = np.ones(n)
+    for i in range(1, n):
+        b[i] = b[i - 1] * (0.5 - (i - 1)) / i
+    h = 1j * sig * np.concatenate([[0.0], b[1:]])
+    if ip == 2:
+        log1p = np.concatenate([[0.0], [(-1) ** (i + 1) / i for i in range(1, n)]])
+        log1m = np.concatenate([[0.0], [-1.0 / i for i in range(1, n)]])
+        h = h - 0.25 * log1p + 2.0 * log1m
+    del k
+    return _series_exp(h, n)
+
+
+def _rational_series(pd1, pd2, n):
+    num = np.array([1.0 + 0j])
+    den = np.array([1.0 + 0j])
+    for p in pd1:
+        num = np.convolve(num, [1.0, p])
+    for p in pd2:
+        den = np.convolve(den, [1.0, p])
+    num = np.concatenate([num, np.zeros(n)])[:n]
+    den = np.concatenate([den, np.zeros(n)])[:n]
+    q = np.zeros(n, complex)
+    for i in range(n):
+        q[i] = (num[i] - np.sum(den[1:i + 1] * q[i - 1::-1][:i])) / den[0]
+    return q
+
+
+@pytest.mark.parametrize("np_pade", [4, 6, 8])
+@pytest.mark.parametrize("ns,ip", [(1, 1), (0, 1), (1, 2), (2, 1)])
+@pytest.mark.parametrize("freq,dr", [(25.0, 10.0), (50.0, 50.0), (100.0, 25.0), (200.0, 100.0)])
+def test_pade_taylor_and_stability(np_pade, ns, ip, freq, dr):
+    c0 = 1500.0
+    pd1, pd2 = O.pade(freq, c0, np_pade, ns, dr, ip)
+    sig = 2 * np.pi * freq / c0 * dr
+    order = 2 * np_pade - ns  # matched Taylor coefficients beyond the constant
+    n = order + 1
+    g = _g_series(sig, ip, n)
+    r = _rational_series(pd1, pd2, n)
+    scale = np.max(np.abs(g))
+    err = np.abs(r - g) / scale
+    assert np.all(err < 1e-9), (err, order)
+    # the next coefficient is not matched in general (guards an off-by-one)
+    g2 = _g_series(sig, ip, n + 1)
+    r2 = _rational_series(pd1, pd2, n + 1)
+    assert abs(r2[-1] - g2[-1]) / scale > 1e3 * max(np.max(err), 1e-16)
+    num = lambda x: np.prod(1 + pd1 * x)
+    if ns >= 1:
+        assert abs(num(-3.0)) < 1e-8 * np.prod(1 + np.abs(pd1) * 3)
+    if ns >= 2:
+        assert abs(num(-1.5)) < 1e-8 * np.prod(1 + np.abs(pd1) * 1.5)
+
+
+def test_pade_product_rule_values():
+    """ns=0 rational function approximates exp(i sig (sqrt(1+X)-1)) at small X."""
+    pd1, pd2 = O.pade(50.0, 1500.0, 8, 0, 50.0, 1)
+    sig = 2 * np.pi * 50 / 1500 * 50
+    for x in (-0.05, 0.03, 0.1j * 0.2):
+        approx = np.prod((1 + pd1 * x) / (1 + pd2 * x))
+        exact = np.exp(1j * sig * (np.sqrt(1 + x) - 1))
+        assert abs(approx - exact) < 1e-9
+
+
+# -- profiles --------------------------------------------------------------------
+
+def test_profiles_match_numpy_interp():
+    kw = envs.pekeris_phase1()
+    ray = O.resolve_ray(**kw)
+    alpw, alpb, rhob, ksqw, ksqb = O.profiles(ray)
+    nz, _, _, _ = ray.dims()
+    z = np.linspace(0, ray.zmax, nz + 2)
+    cw = np.interp(z, ray.z_ss, ray.cw[:, 0])
+    zsb = ray.z_sb + ray.z_ss[-1]
+    cb = np.interp(z, zsb, ray.cb[:, 0])
+    rh = np.interp(z, zsb, ray.rhob[:, 0])
+    lw = ray.lyrw * ray.lam
+    at = np.interp(z, np.concatenate([zsb, [zsb[-1] + 0.75 * lw, zsb[-1] + lw]]),
+                   np.concatenate([ray.attn[:, 0], [ray.attn[-1, 0], 10.0]]))
+    np.testing.assert_allclose(alpw, np.sqrt(cw / ray.c0), rtol=1e-14)
+    np.testing.assert_allclose(alpb, np.sqrt(rh * cb / ray.c0), rtol=1e-14)
+    np.testing.assert_allclose(rhob, rh, rtol=1e-14)
+    omega, k0 = 2 * np.pi * ray.freq, 2 * np.pi * ray.freq / ray.c0
+    eta = 1 / (40 * np.pi * np.log10(np.e))
+    np.testing.assert_allclose(ksqw, (omega / cw) ** 2 - k0 ** 2, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(ksqb, ((omega / cb) * (1 + 1j * eta * at)) ** 2 - k0 ** 2, rtol=1e-12, atol=1e-14)
+
+
+def test_bathymetry_rows_brute_force():
+    kw = envs.wedge()
+    ray = O.resolve_ray(**kw)
+    out = O.run(ray, grids=False)
+    nz, nvr, _, nsteps = ray.dims()
+    bt = ray.rbzb.copy()
+    if bt[-1, 0] < ray.rmax:
+        bt = np.vstack([bt, [ray.rmax, bt[-1, 1]]])
+    expect = []
+    for rn in range(nsteps):
+        r = (rn + 1) * ray.dr
+        seg = max(i for i in range(len(bt) - 1) if r >= bt[i, 0])  # segment chosen by r
+        r0, z0 = bt[seg]
+        r1, z1 = bt[seg + 1]
+        z = z0 + (r + 0.5 * ray.dr - r0) * (z1 - z0) / (r1 - r0)
+        expect.append(min(max(int(np.floor(z / ray.dz)), 1), nz - 1))
+    assert np.array_equal(out["iz"], np.array(expect))
+
+
+# -- elimination identities ------------------------------------------------------
+
+def _medium(nz, seed):
+    rng = np.random.default_rng(seed)
+    alpw = 1 + 0.05 * rng.random(nz + 2)
+    alpb = 1.1 + 0.05 * rng.random(nz + 2)
+    rhob = 1.5 + 0.3 * rng.random(nz + 2)
+    ksqw = (0.01 * rng.normal(size=nz + 2)).astype(complex)
+    ksqb = 0.01 * rng.normal(size=nz + 2) + 0.002j * rng.random(nz + 2)
+    u = rng.normal(size=nz + 2) + 1j * rng.normal(size=nz + 2)
+    u[0] = u[-1] = 0
+    return alpw, alpb, rhob, ksqw, ksqb, u
+
+
+@pytest.mark.parametrize("nz,iz", [(12, 5), (20, 1), (20, 18), (40, 19)])
+def test_elimination_identities(nz, iz):
+    alpw, alpb, rhob, ksqw, ksqb, u = _medium(nz, nz + iz)
+    k0, dz = 0.2, 1.0
+    p = np.array([0.3 - 0.2j])
+    q = np.array([-0.1 + 0.4j])
+    same = O.apply_factors(nz, iz, dz, k0, p, p, alpw, alpb, rhob, ksqw, ksqb, u)
+    np.testing.assert_allclose(same[1:nz + 1], u[1:nz + 1], rtol=1e-11, atol=1e-12)
+    fwd = O.apply_factors(nz, iz, dz, k0, p, q, alpw, alpb, rhob, ksqw, ksqb, u)
+    back = O.apply_factors(nz, iz, dz, k0, q, p, alpw, alpb, rhob, ksqw, ksqb, fwd)
+    np.testing.assert_allclose(back[1:nz + 1], u[1:nz + 1], rtol=1e-10, atol=1e-11)
+    assert np.max(np.abs(fwd - u)) > 1e-3  # the factor is not the identity
+    u2 = np.roll(u, 3)
+    u2[0] = u2[-1] = 0
+    a = O.apply_factors(nz, iz, dz, k0, p, q, alpw, alpb, rhob, ksqw, ksqb, u + 2j * u2)
+    b = O.apply_factors(nz, iz, dz, k0, p, q, alpw, alpb, rhob, ksqw, ksqb, u2)
+    np.testing.assert_allclose(a[1:nz + 1], fwd[1:nz + 1] + 2j * b[1:nz + 1], rtol=1e-10, atol=1e-11)
+
+
+def test_elimination_independent_of_split_row():
+    """PyRAM eliminates towards row iz+1; with the same medium the solution
+    must not depend on where the two sweeps meet."""
+    nz = 30
+    alpw, alpb, rhob, ksqw, ksqb, u = _medium(nz, 3)
+    # identical water and bottom media: iz only moves the meeting row
+    alpb, rhob, ksqb = alpw.copy(), np.ones(nz + 2), ksqw.copy()
+    p = np.array([0.3 - 0.2j, 0.1 + 0.1j])
+    q = np.array([-0.1 + 0.4j, 0.2 - 0.05j])
+    outs = [O.apply_factors(nz, iz, 1.0, 0.2, p, q, alpw, alpb, rhob, ksqw, ksqb, u) for iz in (1, 7, 15, 28)]
+    for o in outs[1:]:
+        np.testing.assert_allclose(o[1:nz + 1], outs[0][1:nz + 1], rtol=1e-11, atol=1e-12)
+
+
+# -- Pekeris normal modes --------------------------------------------------------
+
+def _pekeris_modes(f, D, c1, c2, rho1, rho2, att):
+    from scipy.optimize import brentq
+    omega = 2 * np.pi * f
+    k1 = omega / c1
+    eta = 1 / (40 * np.pi * np.log10(np.e))
+    k2 = omega / c2 * (1 + 1j * eta * att)
+    k2r = omega / c2
+
+    def F(kr, k2v):
+        g1 = np.sqrt(k1 ** 2 - kr ** 2 + 0j)
+        g2 = np.sqrt(kr ** 2 - k2v ** 2 + 0j)
+        if g2.real < 0:
+            g2 = -g2
+        return rho1 * g2 * np.sin(g1 * D) + rho2 * g1 * np.cos(g1 * D)
+
+    grid = np.linspace(k2r * (1 + 1e-9), k1 * (1 - 1e-12), 20001)
+    vals = np.array([F(k, k2r).real for k in grid])
+    roots = [brentq(lambda k: F(k, k2r).real, grid[i], grid[i + 1])
+             for i in range(len(grid) - 1) if vals[i] * vals[i + 1] < 0]
+    modes = []
+    for kr in roots:  # continue into the lossy bottom by Newton in the complex plane
+        z = complex(kr)
+        for _ in range(50):
+            h = 1e-7 * abs(z)
+            d = (F(z + h, k2) - F(z - h, k2)) / (2 * h)
+            step = F(z, k2) / d
+            z -= step
+            if abs(step) < 1e-15:
+                break
+        modes.append(z)
+    return np.array(modes), k1, k2
+
+
+def pekeris_tl(ranges, f, D, c1, c2, rho1, rho2, att, zs, zr):
+    modes, k1, k2 = _pekeris_modes(f, D, c1, c2, rho1, rho2, att)
+    p = np.zeros(len(ranges), complex)
+    for kr in modes:
+        g1 = np.sqrt(k1 ** 2 - kr ** 2)
+        g2 = np.sqrt(kr ** 2 - k2 ** 2)
+        if g2.real < 0:
+            g2 = -g2
+        norm = (D / 2 - np.sin(2 * g1 * D) / (4 * g1)) / rho1 + np.sin(g1 * D) ** 2 / (2 * g2 * rho2)
+        psi = lambda z: np.sin(g1 * z) / np.sqrt(norm)
+        p += psi(zs) * psi(zr) * np.exp(1j * kr * ranges) / np.sqrt(kr)
+    p *= 1j / rho1 / np.sqrt(8 * np.pi * ranges) * np.exp(-1j * np.pi / 4)
+    return -20 * np.log10(np.abs(p) * 4 * np.pi)
+
+
+def test_pekeris_normal_modes():
+    """BASELINE config 1 (100 m isovelocity water over a 1700 m/s, 1.5 g/cm3,
+    0.5 dB/lambda bottom; 50 Hz; zs 25 m, zr 30 m; np 8): the oracle's TL line
+    follows the normal-mode sum to <= 1 dB RMS over 1-5 km."""
+    from cupyram_amd import workloads
+    kw = workloads.pekeris_c1()
+    out = O.run(O.resolve_ray(**kw), grids=False)
+    r = out["Ranges"]
+    tl_nm = pekeris_tl(r, 50.0, 100.0, 1500.0, 1700.0, 1.0, 1.5, 0.5, 25.0, 30.0)
+    sel = r >= 1000.0
+    d = out["TL Line"][sel] - tl_nm[sel]
+    rms = float(np.sqrt(np.mean(d ** 2)))
+    print("Pekeris PE vs modes: rms", rms, "max", np.max(np.abs(d)))
+    assert rms <= 1.0
+    # a wrong bottom density would move the field by several dB
+    tl_bad = pekeris_tl(r, 50.0, 100.0, 1500.0, 1700.0, 1.0, 2.5, 0.5, 25.0, 30.0)
+    assert np.sqrt(np.mean((out["TL Line"][sel] - tl_bad[sel]) ** 2)) > 2 * rms
