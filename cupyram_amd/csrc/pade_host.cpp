@@ -22,6 +22,20 @@ namespace {
 
 using cd = std::complex<double>;
 
+// Complex division by Smith's method (R. L. Smith, CACM 5(8), 1962), written
+// out so the result does not depend on the runtime's complex-division helper.
+cd cdiv(cd a, cd b) {
+  const double ar = a.real(), ai = a.imag(), br = b.real(), bi = b.imag();
+  if (std::fabs(br) >= std::fabs(bi)) {
+    const double t = bi / br, d = br + bi * t;
+    return cd((ar + ai * t) / d, (ai - ar * t) / d);
+  }
+  const double t = br / bi, d = br * t + bi;
+  return cd((ar * t + ai) / d, (ai * t - ar) / d);
+}
+// Product with the textbook formula (no runtime helper).
+cd cmul(cd a, cd b) { return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real()); }
+
 struct PadeWork {
   int np, n;
   double binom[33][33];
@@ -50,7 +64,7 @@ void taylor_of_g(PadeWork &w, double sig, double alp, int nu) {
   w.dg[1] = dsum[0];
   for (int i = 1; i < n; ++i) {
     cd acc = dsum[i];
-    for (int j = 0; j < i; ++j) acc += (w.binom[i][j] * dsum[j]) * w.dg[i - j];
+    for (int j = 0; j < i; ++j) acc += cmul(w.binom[i][j] * dsum[j], w.dg[i - j]);
     w.dg[i + 1] = acc;
   }
 }
@@ -73,18 +87,18 @@ void solve_pivoted(PadeWork &w) {
         for (int j = i; j <= n; ++j) std::swap(w.A[best][j], w.A[i][j]);
       }
     }
-    w.A[i][i] = 1.0 / w.A[i][i];
-    w.rhs[i] *= w.A[i][i];
+    w.A[i][i] = cdiv(1.0, w.A[i][i]);
+    w.rhs[i] = cmul(w.rhs[i], w.A[i][i]);
     if (i < n - 1) {
-      for (int j = i + 1; j <= n; ++j) w.A[i][j] *= w.A[i][i];
+      for (int j = i + 1; j <= n; ++j) w.A[i][j] = cmul(w.A[i][j], w.A[i][i]);
       for (int k = i + 1; k < n; ++k) {
-        w.rhs[k] -= w.A[k][i] * w.rhs[i];
-        for (int j = i + 1; j < n; ++j) w.A[k][j] -= w.A[k][i] * w.A[i][j];
+        w.rhs[k] -= cmul(w.A[k][i], w.rhs[i]);
+        for (int j = i + 1; j < n; ++j) w.A[k][j] -= cmul(w.A[k][i], w.A[i][j]);
       }
     }
   }
   for (int i = n - 2; i >= 0; --i)
-    for (int j = i + 1; j < n; ++j) w.rhs[i] -= w.A[i][j] * w.rhs[j];
+    for (int j = i + 1; j < n; ++j) w.rhs[i] -= cmul(w.A[i][j], w.rhs[j]);
 }
 
 // Laguerre's method on c[0..deg]; false when 100 iterations do not converge.
@@ -95,22 +109,22 @@ bool laguerre(const cd *c, int deg, cd &z, double tol, int max_iter) {
   int it = 0, rot = 0;
   double step = INFINITY;
   while (step > tol && it < max_iter - 1) {
-    cd p = c[deg - 1] + c[deg] * z;
-    for (int i = deg - 2; i >= 0; --i) p = c[i] + z * p;
+    cd p = c[deg - 1] + cmul(c[deg], z);
+    for (int i = deg - 2; i >= 0; --i) p = c[i] + cmul(z, p);
     if (std::abs(p) < 1e-20) break;
-    cd p1 = d1[deg - 2] + d1[deg - 1] * z;
-    for (int i = deg - 3; i >= 0; --i) p1 = d1[i] + z * p1;
-    cd p2 = d2[deg - 3] + d2[deg - 2] * z;
-    for (int i = deg - 4; i >= 0; --i) p2 = d2[i] + z * p2;
-    const cd f = p1 / p;
-    const cd g = f * f - p2 / p;
-    const cd h = std::sqrt(static_cast<double>(deg - 1) * (static_cast<double>(deg) * g - f * f));
-    cd dz = (std::abs(f + h) > std::abs(f - h)) ? -static_cast<double>(deg) / (f + h)
-                                                 : -static_cast<double>(deg) / (f - h);
+    cd p1 = d1[deg - 2] + cmul(d1[deg - 1], z);
+    for (int i = deg - 3; i >= 0; --i) p1 = d1[i] + cmul(z, p1);
+    cd p2 = d2[deg - 3] + cmul(d2[deg - 2], z);
+    for (int i = deg - 4; i >= 0; --i) p2 = d2[i] + cmul(z, p2);
+    const cd f = cdiv(p1, p);
+    const cd g = cmul(f, f) - cdiv(p2, p);
+    const cd h = std::sqrt(static_cast<double>(deg - 1) * (static_cast<double>(deg) * g - cmul(f, f)));
+    cd dz = (std::abs(f + h) > std::abs(f - h)) ? cdiv(-static_cast<double>(deg), f + h)
+                                                 : cdiv(-static_cast<double>(deg), f - h);
     ++it;
     if (++rot == 9) {
       rot = 0;
-      dz *= cd(0.0, 1.0);
+      dz = cd(-dz.imag(), dz.real());
     }
     z += dz;
     step = std::abs(dz);
@@ -121,7 +135,7 @@ bool laguerre(const cd *c, int deg, cd &z, double tol, int max_iter) {
 
 bool poly_roots(cd *c, int deg, cd *z) {
   if (deg == 1) {
-    z[0] = -c[0] / c[1];
+    z[0] = cdiv(-c[0], c[1]);
     return true;
   }
   for (int k = deg - 1; k >= 2; --k) {
@@ -129,12 +143,12 @@ bool poly_roots(cd *c, int deg, cd *z) {
     if (!laguerre(c, k + 1, root, 1e-12, 1000)) return false;
     if (!laguerre(c, k + 1, root, 0.0, 5)) return false;
     z[k] = root;
-    for (int i = k; i >= 0; --i) c[i] += root * c[i + 1];
+    for (int i = k; i >= 0; --i) c[i] += cmul(root, c[i + 1]);
     for (int i = 0; i <= k; ++i) c[i] = c[i + 1];
   }
-  const cd disc = std::sqrt(c[1] * c[1] - 4.0 * c[0] * c[2]);
-  z[1] = 0.5 * (-c[1] + disc) / c[2];
-  z[0] = 0.5 * (-c[1] - disc) / c[2];
+  const cd disc = std::sqrt(cmul(c[1], c[1]) - 4.0 * cmul(c[0], c[2]));
+  z[1] = cdiv(0.5 * (-c[1] + disc), c[2]);
+  z[0] = cdiv(0.5 * (-c[1] - disc), c[2]);
   return true;
 }
 
@@ -181,7 +195,7 @@ int cpr_pade_impl(double freq, double c0, int np_, int ns, double dr, int ip, do
     if (!poly_roots(poly, np_, roots)) return -1;
     double *dst = which == 0 ? pd1 : pd2;
     for (int j = 0; j < np_; ++j) {
-      const cd v = -1.0 / roots[j];
+      const cd v = cdiv(-1.0, roots[j]);
       dst[2 * j] = v.real();
       dst[2 * j + 1] = v.imag();
     }

@@ -26,10 +26,10 @@
  *   orc_run       cupyram/cupyram.py:273-380 + setup :528-726
  *
  * Parity pins (tests/test_oracle.py): RAM v1.5's tl_ref.line (mean |dTL| <=
- * 0.01 dB), Pade closed forms (Taylor match, stability zero, numpy.roots),
- * numpy.interp for profiles, brute-force bathymetry indices, a dense
- * numpy.linalg.solve of the tridiagonal system for the elimination, and a
- * textbook Pekeris normal-mode sum.
+ * 0.01 dB), Pade closed forms (Taylor series by series composition, stability
+ * zeros), numpy.interp for profiles, brute-force bathymetry indices,
+ * identities of the factor solve (identity map, inverse factor, linearity,
+ * independence of the meeting row), and a textbook Pekeris normal-mode sum.
  */
 #include <complex.h>
 #include <math.h>
@@ -43,6 +43,18 @@ typedef double complex cplx;
 /* ------------------------------------------------------------------------ */
 /* Pade coefficients (pade.py:180-273)                                      */
 /* ------------------------------------------------------------------------ */
+
+/* Complex division by Smith's method (R. L. Smith, CACM 5(8), 1962), written
+ * out so the result does not depend on which __divdc3 the process binds. */
+static cplx cdiv(cplx a, cplx b) {
+  const double ar = creal(a), ai = cimag(a), br = creal(b), bi = cimag(b);
+  if (fabs(br) >= fabs(bi)) {
+    const double t = bi / br, d = br + bi * t;
+    return CMPLX((ar + ai * t) / d, (ai - ar * t) / d);
+  }
+  const double t = br / bi, d = br * t + bi;
+  return CMPLX((ar * t + ai) / d, (ai * t - ar) / d);
+}
 
 /* Taylor derivatives of g(x) = exp(i sig (sqrt(1+x)-1)) (1+x)^alp (1-x)^(2nu)
  * at x = 0, via g' = g (h1' + h2' + h3') and Leibniz (pade.py:16-43). */
@@ -96,7 +108,7 @@ static void pade_gauss(int n, cplx *a, cplx *b) {
         }
       }
     }
-    a[i * ld + i] = 1.0 / a[i * ld + i];
+    a[i * ld + i] = cdiv(1.0, a[i * ld + i]);
     b[i] *= a[i * ld + i];
     if (i < n - 1) {
       for (int j = i + 1; j <= n; ++j) a[i * ld + j] *= a[i * ld + i];
@@ -127,10 +139,10 @@ static int pade_laguerre(const cplx *a, int n, cplx *zio, double err, int nter) 
     for (int i = n - 3; i >= 0; --i) pz = az[i] + z * pz;
     cplx pzz = azz[n - 3] + azz[n - 2] * z;
     for (int i = n - 4; i >= 0; --i) pzz = azz[i] + z * pzz;
-    cplx f = pz / p;
-    cplx g = f * f - pzz / p;
+    cplx f = cdiv(pz, p);
+    cplx g = f * f - cdiv(pzz, p);
     cplx h = csqrt((double)(n - 1) * ((double)n * g - f * f));
-    cplx dz = (cabs(f + h) > cabs(f - h)) ? -(double)n / (f + h) : -(double)n / (f - h);
+    cplx dz = (cabs(f + h) > cabs(f - h)) ? cdiv(-(double)n, f + h) : cdiv(-(double)n, f - h);
     ++iter;
     if (++jter == 9) {
       jter = 0;
@@ -148,7 +160,7 @@ static int pade_laguerre(const cplx *a, int n, cplx *zio, double err, int nter) 
  * formula (pade.py:152-177).  a is destroyed. */
 static int pade_roots(cplx *a, int n, cplx *z) {
   if (n == 1) {
-    z[0] = -a[0] / a[1];
+    z[0] = cdiv(-a[0], a[1]);
     return 0;
   }
   if (n != 2) {
@@ -162,8 +174,8 @@ static int pade_roots(cplx *a, int n, cplx *z) {
     }
   }
   cplx disc = csqrt(a[1] * a[1] - 4.0 * a[0] * a[2]);
-  z[1] = 0.5 * (-a[1] + disc) / a[2];
-  z[0] = 0.5 * (-a[1] - disc) / a[2];
+  z[1] = cdiv(0.5 * (-a[1] + disc), a[2]);
+  z[0] = cdiv(0.5 * (-a[1] - disc), a[2]);
   return 0;
 }
 
@@ -219,11 +231,11 @@ int orc_pade(double freq, double c0, int np_, int ns, double dr, int ip,
   poly[0] = 1.0;
   for (int j = 0; j < np_; ++j) poly[j + 1] = b[2 * j];
   if (pade_roots(poly, np_, roots)) return -1;
-  for (int j = 0; j < np_; ++j) pd1[j] = -1.0 / roots[j];
+  for (int j = 0; j < np_; ++j) pd1[j] = cdiv(-1.0, roots[j]);
   poly[0] = 1.0;
   for (int j = 0; j < np_; ++j) poly[j + 1] = b[2 * j + 1];
   if (pade_roots(poly, np_, roots)) return -1;
-  for (int j = 0; j < np_; ++j) pd2[j] = -1.0 / roots[j];
+  for (int j = 0; j < np_; ++j) pd2[j] = cdiv(-1.0, roots[j]);
   return 0;
 }
 

@@ -1,0 +1,149 @@
+"""Host-side logic and the C ABI surface, on the CPU (no GPU calls)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cupyram_amd import AsyncBatcher, CuPyRAM, _abi, compute_pade_coefficients
+from cupyram_amd.pade import compute_pade_batch
+from oracle import ram_oracle as O
+from tests import envs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "cupyram_hip.h")).read()
+    declared = set(re.findall(r"\b(cpr_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) >= 15
+    lib = _abi.lib()
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert declared <= set(_abi._SYMBOLS)
+
+
+def test_plan_create_without_device_fails_loudly():
+    if _abi.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    m = CuPyRAM(**envs.pekeris_batching())
+    with pytest.raises(_abi.CprError, match="no HIP device"):
+        m.run()
+
+
+@pytest.mark.parametrize("np_pade", [4, 6, 8])
+@pytest.mark.parametrize("ns,ip", [(1, 1), (0, 1), (1, 2)])
+@pytest.mark.parametrize("freq", [25.0, 50.0, 100.0, 200.0])
+@pytest.mark.parametrize("dr", [10.0, 25.0, 50.0, 100.0])
+def test_pade_matches_oracle(np_pade, ns, ip, freq, dr):
+    """The reference's Pade grid and 1e-8 gate (tests/test_pade_coefficients.py:23-68)."""
+    a1, a2 = compute_pade_coefficients(freq, 1500.0, np_pade, ns, dr, ip)
+    b1, b2 = O.pade(freq, 1500.0, np_pade, ns, dr, ip)
+    assert np.max(np.abs(a1 - b1)) < 1e-8 and np.max(np.abs(a2 - b2)) < 1e-8
+
+
+def test_pade_batch_equals_scalar():
+    f = np.array([25.0, 50.0, 100.0])
+    c0 = np.array([1480.0, 1500.0, 1520.0])
+    out = compute_pade_batch(f, c0, 6, 1, 10.0, 1, nthreads=2)
+    for i in range(3):
+        p1, p2 = compute_pade_coefficients(f[i], c0[i], 6, 1, 10.0, 1)
+        assert np.array_equal(out[i, 0], p1) and np.array_equal(out[i, 1], p2)
+
+
+def test_pade_rejects_bad_arguments():
+    with pytest.raises(ValueError):
+        compute_pade_coefficients(50.0, 1500.0, 0, 1, 10.0, 1)
+    with pytest.raises(ValueError):
+        compute_pade_coefficients(-5.0, 1500.0, 6, 1, 10.0, 1)
+
+
+def test_normalize_ragged_lists_pads_with_nan():
+    x = CuPyRAM._normalize_to_batch([np.array([1.0, 2.0]), np.array([3.0, 4.0, 5.0])], 2, "x")
+    assert x.shape == (2, 3) and np.isnan(x[0, 2]) and x[1, 2] == 5.0
+    t = CuPyRAM._normalize_to_batch(np.array([1.0, 2.0]), 3, "t")
+    assert t.shape == (3, 2)
+    s = CuPyRAM._normalize_to_batch(7.0, 4, "s")
+    assert s.shape == (4,)
+    with pytest.raises(ValueError, match="list length"):
+        CuPyRAM._normalize_to_batch([np.zeros(2)], 2, "bad")
+
+
+def test_valid_slice():
+    a, n = CuPyRAM._get_valid_slice(np.array([1.0, 2.0, np.nan]))
+    assert n == 2 and a.tolist() == [1.0, 2.0]
+    b = np.full((3, 4), np.nan)
+    b[:2, :3] = 1.0
+    v, n = CuPyRAM._get_valid_slice(b)
+    assert v.shape == (2, 3) and n == 2
+
+
+def test_input_validation_errors():
+    kw = envs.pekeris_batching()
+    with pytest.raises(ValueError, match="Source depth"):
+        CuPyRAM(**dict(kw, zs=500.0))
+    with pytest.raises(ValueError, match="Receiver depth"):
+        CuPyRAM(**dict(kw, zr=-1.0))
+    with pytest.raises(ValueError, match="inconsistent"):
+        CuPyRAM(**dict(kw, cw=np.array([[1500.0, 1500.0], [1500.0, 1500.0]])))
+    with pytest.raises(ValueError, match="Max bathy"):
+        CuPyRAM(**dict(kw, rbzb=np.array([[0.0, 300.0], [10000.0, 300.0]])))
+    with pytest.raises(ValueError, match="np must be"):
+        CuPyRAM(**dict(kw, np=40))
+
+
+def test_default_grid_uses_max_frequency():
+    """tests/test_multi_frequency.py:205-230: dr = 60 m, dz = 0.75 m for f = [50, 100, 200]."""
+    kw = dict(envs.pekeris_batching(), freq=[50.0, 100.0, 200.0])
+    kw.pop("rmax")
+    m = CuPyRAM(**kw, rmax=1000.0)
+    assert m._dr == pytest.approx(60.0) and m._dz == pytest.approx(0.75)
+
+
+@pytest.mark.parametrize("name", ["ram_reference_case", "pekeris_batching", "pekeris_phase1", "wedge",
+                                  "varying_ranges", "munk_batching"])
+def test_setup_grid_matches_oracle(name):
+    m = CuPyRAM(**getattr(envs, name)())
+    m.setup()
+    for e in range(m._n_env):
+        nz, nvr, nvz, nsteps = envs.oracle_for(m, e).dims()
+        r = m._rays[e]
+        assert (r["nz"], r["nvr"], r["nvz"], r["nsteps"]) == (nz, nvr, nvz, nsteps)
+    # attributes the reference's tests read after run()/setup()
+    assert m.nz == int(m._rays["nz"].max()) and m._dz > 0 and m._dr > 0
+    assert len(m.vr) == m._nvr
+
+
+def test_per_ray_grids():
+    kw = envs.varying_ranges()
+    kw["dz"] = np.array([1.0, 2.0, 1.5])
+    kw["dr"] = np.array([25.0, 50.0, 40.0])
+    m = CuPyRAM(**kw)
+    m.setup()
+    assert m._per_ray_grid
+    assert len(set(m._rays["nz"])) == 3
+    for e in range(3):
+        nz, nvr, nvz, nsteps = envs.oracle_for(m, e).dims()
+        r = m._rays[e]
+        assert (r["nz"], r["nvr"], r["nsteps"]) == (nz, nvr, nsteps)
+
+
+def _square(x):
+    return x * x
+
+
+def test_async_batcher_order_and_prefetch():
+    got = list(AsyncBatcher(((i,) for i in range(7)), _square, max_prefetch=3, max_workers=2))
+    assert got == [i * i for i in range(7)]
+    assert list(AsyncBatcher(iter([]), _square)) == []
+
+
+def test_shared_object_roundtrip():
+    obj = {"a": np.arange(5), "b": "text"}
+    name, size = AsyncBatcher.share_object(obj)
+    try:
+        back = AsyncBatcher.load_shared_object(name, size)
+        assert back["b"] == "text" and np.array_equal(back["a"], obj["a"])
+    finally:
+        AsyncBatcher.cleanup_shared_object(name)
+    AsyncBatcher.cleanup_shared_object(name)  # second cleanup is a no-op
