@@ -42,8 +42,10 @@ def time_march(model, steps, warmup, launch_steps):
     """Selfs + warmup untimed, then `steps` timed march steps."""
     import cupyram_amd  # noqa: F401
     model.setup()
-    nc = model._total_batch
-    plan = model._make_plan(np.arange(nc))
+    buckets = model._buckets()
+    assert len(buckets) == 1, "the timed workload has one grid"
+    idx, shape, variant = buckets[0]
+    plan = model._make_plan(idx, shape=shape, variant=variant)
     try:
         plan.selfs()
         if warmup:
@@ -104,6 +106,62 @@ def cpu_baseline(kw, sample_rays, sample_rmax, precision_runs):
                       f"OpenMP over rays ({cell_steps:.3g} cell-steps in {t:.1f} s)"}, errs
 
 
+def bench_uneven(args):
+    """The uneven batch (per-ray frequency and grid): the whole job after the
+    self-starter, every nz-bucket on its own stream, march launches of
+    --launch-steps range steps captured in one hipGraph per bucket."""
+    from cupyram_amd import CuPyRAM, workloads
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+    kw = workloads.uneven_batch(nrays=args.rays)
+    log(f"uneven: generated {args.rays} rays")
+    out = {}
+    for prec in (("c64", "c128") if args.with_c128 else ("c64",)):
+        m = CuPyRAM(**kw, precision=prec)
+        m.setup()
+        buckets = m._buckets()
+        log(f"uneven {prec}: setup done, {len(buckets)} buckets")
+        plans = []
+        try:
+            for idx, shape, variant in buckets:
+                plans.append((idx, m._make_plan(idx, shape=shape, variant=variant)))
+            for _, p in plans:
+                p.selfs()
+            for _, p in plans:
+                p.sync()
+            log(f"uneven {prec}: self-starters done, marching")
+            t0 = time.perf_counter()
+            for _, p in plans:
+                p.march_graph(args.launch_steps)
+            for i, (_, p) in enumerate(plans):
+                p.sync()
+                log(f"uneven {prec}: bucket {i} done at {time.perf_counter() - t0:.1f} s")
+            wall = time.perf_counter() - t0
+            per_bucket = []
+            for idx, p in plans:
+                ms, n = p.march_time()
+                v, r, l = p.describe(0)
+                cs = float(np.sum(m._rays["nz"][idx].astype(np.float64) * m._rays["nsteps"][idx]))
+                per_bucket.append({"rays": len(idx), "variant": v, "rows": r, "lanes": l,
+                                   "nz_max": int(m._rays["nz"][idx].max()), "kernel_s": ms / 1e3,
+                                   "cell_steps": cs, "mcs": cs / (ms / 1e3) / 1e6 if ms else None})
+        finally:
+            for _, p in plans:
+                p.close()
+        cells = float(np.sum(m._rays["nz"].astype(np.float64) * m._rays["nsteps"]))
+        out[prec] = {"value": cells / wall / 1e6, "wall_s": wall, "cell_steps": cells, "buckets": per_bucket}
+    line = {"metric": METRIC, "value": out["c64"]["value"], "unit": "Mcell-steps/s", "n_gpus": 1,
+            "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32 (complex64 march; FP64 Pade and self-starter)",
+            "data": "synthetic (seeded uneven batch)",
+            "config": {"workload": f"uneven_batch: {args.rays} rays, depth logU 100-5000 m, freq logU 25-500 Hz, "
+                                   "dz=lambda/10, dr=lambda/2, rmax 20 km, np 6 (whole job)"},
+            "c64": out["c64"], "c128": out.get("c128")}
+    if "c128" in out:
+        line["speedup_c64_vs_c128"] = out["c64"]["value"] / out["c128"]["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,14 +171,31 @@ def main(argv=None):
     ap.add_argument("--variant", default="auto")
     ap.add_argument("--launch-steps", type=int, default=100, help="range steps per kernel launch")
     ap.add_argument("--no-c128", action="store_true")
+    ap.add_argument("--with-c128", action="store_true", help="uneven workload: also march in complex128")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--cpu-rmax", type=float, default=5000.0)
+    ap.add_argument("--workload", default="munk", choices=["munk", "uneven"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device (torch) so that HBM counters can be scaled")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(rays are independent: shard the ray list across processes)"}))
         return 2
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.workload == "uneven":
+        if args.rays == 10000:
+            args.rays = 50000
+        if args.launch_steps == 100:
+            args.launch_steps = 1000
+        return bench_uneven(args)
     from cupyram_amd import CuPyRAM, workloads
     kw = workloads.munk_batch(nrays=args.rays)
     np_pade = kw["np"]

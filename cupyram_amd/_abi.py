@@ -81,6 +81,7 @@ _SYMBOLS = {
     "cpr_read_outputs": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "cpr_read_field": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     "cpr_plan_device_bytes": ([C.c_void_p], C.c_int64),
+    "cpr_team_shape": ([C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "cpr_plan_describe": ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
                           C.c_int),
 }
@@ -116,6 +117,12 @@ def device_count() -> int:
     n = C.c_int(0)
     lib().cpr_device_count(C.byref(n))
     return n.value
+
+
+def team_shape(precision: int, nz: int):
+    r, l = C.c_int(), C.c_int()
+    check(lib().cpr_team_shape(int(precision), int(nz), C.byref(r), C.byref(l)))
+    return r.value, l.value
 
 
 def ptr(a: np.ndarray):

@@ -21,6 +21,8 @@ __global__ void k_tl_post(MarchArgs a);
 int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, int s1, hipStream_t st);
 bool team_shape_supported(int precision, int rows, int lanes);
 void team_auto_shape(int precision, int nz, int *rows, int *lanes);
+int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays);
+int64_t team_xbuf_bytes(int precision, int rows, int lanes, int nrays);
 
 namespace {
 
@@ -54,6 +56,8 @@ struct cpr_plan {
   void *col[7] = {};  // f1 f2 f3 ksq u cp dp
   double *tll_d = nullptr, *cpl_d = nullptr, *tlg_d = nullptr, *cpg_d = nullptr;
   int32_t *team_rays_d = nullptr;
+  char *dbuf_d = nullptr;
+  char *xbuf_d = nullptr;
   int variant = CPR_VARIANT_THREAD;
   int team_rows = 0, team_lanes = 0;
   std::vector<hipEvent_t> events;  // pairs around march launches
@@ -87,6 +91,8 @@ struct cpr_plan {
     a.cpg = cpg_d;
     a.team_rays = team_rays_d;
     a.team_nrays = nrays;
+    a.dbuf = dbuf_d;
+    a.xbuf = xbuf_d;
     return a;
   }
   size_t real_size() const { return cfg.precision == CPR_C64 ? sizeof(float) : sizeof(double); }
@@ -198,7 +204,8 @@ int cpr_plan_destroy(cpr_plan *p) {
   for (void *q : {static_cast<void *>(p->rays_d), static_cast<void *>(p->envs_d), static_cast<void *>(p->env_d),
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
                   static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d),
-                  static_cast<void *>(p->team_rays_d)})
+                  static_cast<void *>(p->team_rays_d), static_cast<void *>(p->dbuf_d),
+                  static_cast<void *>(p->xbuf_d)})
     if (q) (void)hipFree(q);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -273,6 +280,10 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     std::vector<int32_t> ids(cfg->nrays);
     for (int b = 0; b < cfg->nrays; ++b) ids[b] = b;
     if (dalloc(p, &p->team_rays_d, cfg->nrays)) return bail(-1);
+    const int64_t db = team_dbuf_bytes(cfg->precision, p->team_rows, p->team_lanes, cfg->nrays);
+    if (db > 0 && dalloc(p, &p->dbuf_d, static_cast<size_t>(db))) return bail(-1);
+    const int64_t xb = team_xbuf_bytes(cfg->precision, p->team_rows, p->team_lanes, cfg->nrays);
+    if (xb > 0 && dalloc(p, &p->xbuf_d, static_cast<size_t>(xb))) return bail(-1);
     if (hipMemcpy(p->team_rays_d, ids.data(), sizeof(int32_t) * cfg->nrays, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail("upload failed"));
   }
@@ -493,6 +504,16 @@ int cpr_read_field(cpr_plan *p, double *u, int nz_max) {
 }
 
 int64_t cpr_plan_device_bytes(cpr_plan *p) { return p ? p->bytes : -1; }
+
+int cpr_team_shape(int precision, int nz, int *rows, int *lanes) {
+  if (precision != CPR_C64 && precision != CPR_C128) return fail("precision must be 64 or 128");
+  int r = 0, l = 0;
+  team_auto_shape(precision, nz, &r, &l);
+  if (r <= 0 || static_cast<int64_t>(r) * l < nz) r = l = 0;
+  if (rows) *rows = r;
+  if (lanes) *lanes = l;
+  return 0;
+}
 
 int cpr_plan_describe(cpr_plan *p, int b, int *variant, int *team_rows, int *team_lanes) {
   if (!p || b < 0 || b >= p->nrays) return fail("bad plan or ray");

@@ -98,22 +98,30 @@ CPR_HD inline double alpw_at(const cpr_ray &P, const cpr_env &E, const double *e
   return sqrt(water_speed(E, env, ss, z) / P.c0);
 }
 
-// Medium of one grid row for water/bottom selection by iz (FP64).
-CPR_HD inline MediumRow medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int ss, int sb, int iz,
-                                  int row) {
+// Water medium of one grid row (FP64): f1 = 1/alpw, f2 = 1, f3 = alpw,
+// ksq = (omega/cw)^2 - k0^2.
+CPR_HD inline MediumRow water_medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int ss, int row) {
+  const double z = grid_depth(P, row);
+  MediumRow m;
+  const double cw = water_speed(E, env, ss, z);
+  const double a = sqrt(cw / P.c0);
+  m.f1 = 1.0 / a;
+  m.f2 = 1.0;
+  m.f3 = a;
+  const double ow = P.omega / cw;
+  m.kre = ow * ow - P.k0 * P.k0;
+  m.kim = 0.0;
+  return m;
+}
+
+// Bottom medium of one grid row (FP64): f1 = rhob/alpb, f2 = 1/rhob,
+// f3 = alpb, ksq = ((omega/cb)(1 + i eta attn))^2 - k0^2 with the absorbing
+// layer ramp below the last seabed depth.
+CPR_HD inline MediumRow bottom_medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int sb, int row) {
   const double z = grid_depth(P, row);
   MediumRow m;
   const double k0sq = P.k0 * P.k0;
-  if (row <= iz) {
-    const double cw = water_speed(E, env, ss, z);
-    const double a = sqrt(cw / P.c0);
-    m.f1 = 1.0 / a;
-    m.f2 = 1.0;
-    m.f3 = a;
-    const double ow = P.omega / cw;
-    m.kre = ow * ow - k0sq;
-    m.kim = 0.0;
-  } else {
+  {
     const double *zsb = env + E.z_sb;
     const int nsb = E.n_sb, st = E.n_rp_sb;
     const double cb = interp_clamped(zsb, env + E.cb + sb, nsb, st, z);
@@ -140,6 +148,12 @@ CPR_HD inline MediumRow medium_at(const cpr_ray &P, const cpr_env &E, const doub
     m.kim = 2.0 * tr * ti;
   }
   return m;
+}
+
+// Medium of one grid row for water/bottom selection by iz (FP64).
+CPR_HD inline MediumRow medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int ss, int sb, int iz,
+                                  int row) {
+  return row <= iz ? water_medium_at(P, E, env, ss, row) : bottom_medium_at(P, E, env, sb, row);
 }
 
 // Range update of one ray before a march step at range r.  Returns a bit

@@ -1,5 +1,6 @@
-// Team-of-lanes march for gfx950: the whole column of a ray stays in
-// registers for every range step of a launch.
+// Team-of-lanes march for gfx950: the field of a ray stays in registers for
+// every range step of a launch (and, for columns up to 12 rows per lane in
+// complex64 / 6 in complex128, its Galerkin coefficients too).
 //
 // A team of T lanes owns one ray; lane t holds rows g = 1 + t*M .. (t+1)*M
 // (M rows per lane, a template parameter, so every per-row array is fully
@@ -150,56 +151,137 @@ struct Team {
 
 template <class R, int M>
 struct LaneCol {
-  cx<R> d1[M], d2[M], d3[M];  // Galerkin coefficients of the lane's rows
-  cx<R> u[M];                 // field, then d', then D
-  cx<R> cw[M];                // c', then C
-  cx<R> ww[M];                // w, then W
+  cx<R> u[M];  // field, then d', then D
 };
 
-// Rebuild the Galerkin coefficients of the lane's rows from the medium.  The
-// FP64 medium of rows g0-1 .. g0+M is evaluated in a rolled loop with a
-// three-row window, the coefficients land in a small private array (this
-// runs only when the medium changed) and are then copied into the register
-// rows.  f3 of every row is kept for the TL output.
+// Elimination arrays of a lane: c' (then C) and the spike w (then W) of
+// every row, in VGPRs (DX = false) or in a global buffer laid out
+// [ray][row k][lane][2] (DX = true: only the field stays in VGPRs, which
+// lets one lane hold 40-64 complex64 rows of the deepest columns).
+template <class R, int M, bool DX>
+struct Elim {
+  cx<R> cw[M], ww[M];
+  __device__ cx<R> c(int k) const { return cw[k]; }
+  __device__ cx<R> w(int k) const { return ww[k]; }
+  __device__ void set(int k, cx<R> cv, cx<R> wv) {
+    cw[k] = cv;
+    ww[k] = wv;
+  }
+};
 template <class R, int M>
-__device__ void rebuild_rows(LaneCol<R, M> &L, R (&f3row)[M], const cpr_ray &P, const cpr_env &E,
-                             const double *env, const RayState &st, int g0) {
-  R dd[M][6], ff[M];
-  const double cfact = 0.5 / (P.dz * P.dz);
-  const double dfact = 1.0 / 12.0;
-  MediumRow w0, w1;
+struct Elim<R, M, true> {
+  cx<R> *base;  // row k at base[2 * k * T] (c) and base[2 * k * T + 1] (w)
+  int T;
+  __device__ cx<R> c(int k) const { return base[2 * static_cast<int64_t>(k) * T]; }
+  __device__ cx<R> w(int k) const { return base[2 * static_cast<int64_t>(k) * T + 1]; }
+  __device__ void set(int k, cx<R> cv, cx<R> wv) {
+    base[2 * static_cast<int64_t>(k) * T] = cv;
+    base[2 * static_cast<int64_t>(k) * T + 1] = wv;
+  }
+};
+
+// Galerkin coefficients of one row.
+template <class R>
+struct CoefRow {
+  cx<R> d1, d2, d3;
+};
+
+// Where a lane keeps the coefficients of its rows: VGPRs (DS = false), or a
+// global buffer laid out [ray][row k][lane] so that a wavefront reading row k
+// touches one contiguous span (DS = true, "d-stream": frees 7 VGPRs per row
+// for deep columns; the buffer of the rays in flight stays L2/MALL-resident).
+template <class R, int M, bool DS>
+struct Coefs {
+  CoefRow<R> r[M];
+  __device__ CoefRow<R> get(int k) const { return r[k]; }
+};
+template <class R, int M>
+struct Coefs<R, M, true> {
+  CoefRow<R> *base;  // row k at base[k * T]
+  int T;
+  __device__ CoefRow<R> get(int k) const { return base[static_cast<int64_t>(k) * T]; }
+};
+
+// Water and bottom media of the lane's rows g0-1 .. g0+M (FP64), kept in a
+// per-lane private array.  Both sides are evaluated when the profiles change
+// (once per launch and at profile range switches); when only the seabed row
+// iz moves, the coefficients of the rows it touches are re-formed from this
+// cache without interpolating the profiles again.
+template <int M>
+struct MedCache {
+  double w[M + 2][5];
+  double b[M + 2][5];
+};
+
+__device__ inline void put_med(double (&d)[5], const MediumRow &m) {
+  d[0] = m.f1;
+  d[1] = m.f2;
+  d[2] = m.f3;
+  d[3] = m.kre;
+  d[4] = m.kim;
+}
+
+template <int M>
+__device__ void fill_media(MedCache<M> &mc, const cpr_ray &P, const cpr_env &E, const double *env,
+                           const RayState &st, int g0) {
 #pragma unroll 1
   for (int q = 0; q < M + 2; ++q) {
     const int row = g0 - 1 + q;
-    MediumRow m;
     if (row <= P.nz + 1) {
-      m = medium_at(P, E, env, st.ss_ind, st.sb_ind, st.iz, row);
+      put_med(mc.w[q], water_medium_at(P, E, env, st.ss_ind, row));
+      put_med(mc.b[q], bottom_medium_at(P, E, env, st.sb_ind, row));
     } else {
+      MediumRow m;
       m.f1 = m.f2 = m.f3 = 1.0;
       m.kre = m.kim = 0.0;
+      put_med(mc.w[q], m);
+      put_med(mc.b[q], m);
     }
-    if (q >= 2) {
-      const int k = q - 2;  // row g0+k uses the medium of rows g0+k-1 .. g0+k+1
-      const double c1 = cfact * w1.f1 * (w0.f2 + w1.f2) * w0.f3;
-      const double c2 = -cfact * w1.f1 * (w0.f2 + 2.0 * w1.f2 + m.f2) * w1.f3;
-      const double c3 = cfact * w1.f1 * (w1.f2 + m.f2) * m.f3;
-      dd[k][0] = static_cast<R>(c1 + dfact * (w0.kre + w1.kre));
-      dd[k][1] = static_cast<R>(dfact * (w0.kim + w1.kim));
-      dd[k][2] = static_cast<R>(c2 + dfact * (w0.kre + 6.0 * w1.kre + m.kre));
-      dd[k][3] = static_cast<R>(dfact * (w0.kim + 6.0 * w1.kim + m.kim));
-      dd[k][4] = static_cast<R>(c3 + dfact * (w1.kre + m.kre));
-      dd[k][5] = static_cast<R>(dfact * (w1.kim + m.kim));
-      ff[k] = static_cast<R>(w1.f3);
-    }
-    w0 = w1;
-    w1 = m;
   }
+}
+
+// Coefficients of lane row k (global row g0+k) from the cached media, water
+// for rows <= iz (matrc.py:63-107) and the Galerkin stencil of
+// fused_kernel.py:43-58.
+template <class R, int M>
+__device__ CoefRow<R> coef_from_cache(const MedCache<M> &mc, int k, int g0, int iz, double cfact) {
+  const double dfact = 1.0 / 12.0;
+  const double *m0 = (g0 + k - 1 <= iz) ? mc.w[k] : mc.b[k];
+  const double *m1 = (g0 + k <= iz) ? mc.w[k + 1] : mc.b[k + 1];
+  const double *m2 = (g0 + k + 1 <= iz) ? mc.w[k + 2] : mc.b[k + 2];
+  const double c1 = cfact * m1[0] * (m0[1] + m1[1]) * m0[2];
+  const double c2 = -cfact * m1[0] * (m0[1] + 2.0 * m1[1] + m2[1]) * m1[2];
+  const double c3 = cfact * m1[0] * (m1[1] + m2[1]) * m2[2];
+  CoefRow<R> cr;
+  cr.d1 = cxm<R>(static_cast<R>(c1 + dfact * (m0[3] + m1[3])), static_cast<R>(dfact * (m0[4] + m1[4])));
+  cr.d2 = cxm<R>(static_cast<R>(c2 + dfact * (m0[3] + 6.0 * m1[3] + m2[3])),
+                 static_cast<R>(dfact * (m0[4] + 6.0 * m1[4] + m2[4])));
+  cr.d3 = cxm<R>(static_cast<R>(c3 + dfact * (m1[3] + m2[3])), static_cast<R>(dfact * (m1[4] + m2[4])));
+  return cr;
+}
+
+// f3 (alpw or alpb) of lane row k from the cached media, for the TL output.
+template <int M>
+__device__ inline double f3_from_cache(const MedCache<M> &mc, int k, int g0, int iz) {
+  return (g0 + k <= iz) ? mc.w[k + 1][2] : mc.b[k + 1][2];
+}
+
+// Re-form the coefficients of lane rows klo..khi (clamped to the lane).
+template <class R, int M, bool DS>
+__device__ void rebuild_rows(Coefs<R, M, DS> &C, const MedCache<M> &mc, int g0, int iz, double cfact, int klo,
+                             int khi) {
+  klo = klo < 0 ? 0 : klo;
+  khi = khi > M - 1 ? M - 1 : khi;
+  if constexpr (DS) {
+#pragma unroll 1
+    for (int k = klo; k <= khi; ++k) C.base[static_cast<int64_t>(k) * C.T] = coef_from_cache<R, M>(mc, k, g0, iz, cfact);
+  } else {
+    CoefRow<R> tmp[M];
+#pragma unroll 1
+    for (int k = klo; k <= khi; ++k) tmp[k] = coef_from_cache<R, M>(mc, k, g0, iz, cfact);
 #pragma unroll
-  for (int k = 0; k < M; ++k) {
-    L.d1[k] = cxm<R>(dd[k][0], dd[k][1]);
-    L.d2[k] = cxm<R>(dd[k][2], dd[k][3]);
-    L.d3[k] = cxm<R>(dd[k][4], dd[k][5]);
-    f3row[k] = ff[k];
+    for (int k = 0; k < M; ++k)
+      if (k >= klo && k <= khi) C.r[k] = tmp[k];
   }
 }
 
@@ -212,16 +294,7 @@ __device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
   return r;
 }
 
-template <class R, int M>
-__device__ inline R pickr(const R (&v)[M], int k) {
-  R r = R(0);
-#pragma unroll
-  for (int q = 0; q < M; ++q)
-    if (q == k) r = v[q];
-  return r;
-}
-
-template <class R, int M, int TW>
+template <class R, int M, int TW, bool DS, bool DX>
 __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *lds, int s0, int s1) {
   const cpr_ray P = make_uniform(a.rays[b]);
   const cpr_env E = make_uniform(a.envs[b]);
@@ -232,7 +305,18 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   int mv = nz - g0 + 1;  // valid rows of this lane
   mv = mv < 0 ? 0 : (mv > M ? M : mv);
   LaneCol<R, M> L;
-  R f3row[M];
+  Coefs<R, M, DS> C;
+  Elim<R, M, DX> X;
+  if constexpr (DX) {
+    X.base = static_cast<cx<R> *>(a.xbuf) + 2 * (static_cast<int64_t>(b) * M * T + t);
+    X.T = T;
+  }
+  MedCache<M> mc;
+  const double cfact = 0.5 / (P.dz * P.dz);
+  if constexpr (DS) {
+    C.base = static_cast<CoefRow<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T + t;
+    C.T = T;
+  }
   cx<R> *ug = static_cast<cx<R> *>(a.u);
 #pragma unroll
   for (int k = 0; k < M; ++k) L.u[k] = k < mv ? ug[static_cast<int64_t>(g0 + k) * a.ld + b] : cxm<R>(0, 0);
@@ -251,8 +335,12 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
     st = make_uniform(st);
     // f changes on rows (lo, hi] when iz moves; d changes on rows lo .. hi+1
     const int lo = jz < st.iz ? jz : st.iz, hi = jz > st.iz ? jz : st.iz;
-    const bool touched = (fl & 1) && g0 <= hi + 1 && g0 + M - 1 >= lo;
-    if (fresh || (fl & 2) || touched) rebuild_rows<R, M>(L, f3row, P, E, a.env, st, g0);
+    if (fresh || (fl & 2)) {
+      fill_media<M>(mc, P, E, a.env, st, g0);
+      rebuild_rows<R, M, DS>(C, mc, g0, st.iz, cfact, 0, M - 1);
+    } else if ((fl & 1) && g0 <= hi + 1 && g0 + M - 1 >= lo) {
+      rebuild_rows<R, M, DS>(C, mc, g0, st.iz, cfact, lo - g0, hi + 1 - g0);
+    }
     fresh = false;
     const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
     for (int j = 0; j < P.np; ++j) {
@@ -266,10 +354,11 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
       for (int k = 0; k < M; ++k) {
         const cx<R> uc = L.u[k];
         const cx<R> up = (k + 1 < M) ? L.u[k + 1] : u_next;
-        const cx<R> r1 = cxm<R>(a1, 0) + p2 * L.d1[k];
-        const cx<R> r2 = cxm<R>(a2, 0) + p2 * L.d2[k];
-        const cx<R> r3 = cxm<R>(a1, 0) + p2 * L.d3[k];
-        const cx<R> du = L.d1[k] * um + L.d2[k] * uc + L.d3[k] * up;
+        const CoefRow<R> cr = C.get(k);
+        const cx<R> r1 = cxm<R>(a1, 0) + p2 * cr.d1;
+        const cx<R> r2 = cxm<R>(a2, 0) + p2 * cr.d2;
+        const cx<R> r3 = cxm<R>(a1, 0) + p2 * cr.d3;
+        const cx<R> du = cr.d1 * um + cr.d2 * uc + cr.d3 * up;
         const cx<R> rhs = a1 * (um + up) + a2 * uc + p1 * du;
         const cx<R> inv = fast_rcp(cfms(r2, r1, cprev));
         cx<R> cc = r3 * inv;
@@ -278,8 +367,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
         if (k >= mv) {
           cc = wk = dk = cxm<R>(0, 0);
         }
-        L.cw[k] = cc;
-        L.ww[k] = wk;
+        X.set(k, cc, wk);
         L.u[k] = dk;
         cprev = cc;
         wprev = wk;
@@ -289,15 +377,15 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
       // 2. upward sweep: x_k = D_k - W_k y_{t-1} - C_k y_t for k < M-1
 #pragma unroll
       for (int k = M - 3; k >= 0; --k) {
-        L.u[k] = cfms(L.u[k], L.cw[k], L.u[k + 1]);
-        L.ww[k] = cfms(L.ww[k], L.cw[k], L.ww[k + 1]);
-        L.cw[k] = cxm<R>(0, 0) - L.cw[k] * L.cw[k + 1];
+        const cx<R> ck = X.c(k);
+        L.u[k] = cfms(L.u[k], ck, L.u[k + 1]);
+        X.set(k, cxm<R>(0, 0) - ck * X.c(k + 1), cfms(X.w(k), ck, X.w(k + 1)));
       }
       // 3. reduced equation a y_{t-1} + b y_t + c y_{t+1} = f, then PCR
       cx<R> nD, nW, nC;
-      team.next3(L.u[0], L.ww[0], L.cw[0], nD, nW, nC);
-      const cx<R> ce = L.cw[M - 1];
-      cx<R> ea = t > 0 ? L.ww[M - 1] : cxm<R>(0, 0);
+      team.next3(L.u[0], X.w(0), X.c(0), nD, nW, nC);
+      const cx<R> ce = X.c(M - 1);
+      cx<R> ea = t > 0 ? X.w(M - 1) : cxm<R>(0, 0);
       cx<R> eb = cxm<R>(1, 0) - ce * nW;
       cx<R> ec = cxm<R>(0, 0) - ce * nC;
       cx<R> ef = cfms(L.u[M - 1], ce, nD);
@@ -324,7 +412,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
       const cx<R> yl = team.up1(yc, 1);
 #pragma unroll
       for (int k = 0; k < M - 1; ++k) {
-        const cx<R> x = L.u[k] - L.ww[k] * yl - L.cw[k] * yc;
+        const cx<R> x = L.u[k] - X.w(k) * yl - X.c(k) * yc;
         L.u[k] = k < mv ? x : cxm<R>(0, 0);
       }
       L.u[M - 1] = (M - 1) < mv ? yc : cxm<R>(0, 0);
@@ -339,7 +427,8 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
         const int ta = ir >= 1 ? (ir - 1) / M : 0, ka = ir >= 1 ? (ir - 1) % M : 0;
         const int tb = ir / M, kb = ir % M;
         cx<R> va = pick<R, M>(L.u, ka), vb = pick<R, M>(L.u, kb);
-        R fa = pickr<R, M>(f3row, ka), fb = pickr<R, M>(f3row, kb);
+        const R fa = static_cast<R>(f3_from_cache<M>(mc, ka, g0, st.iz));
+        const R fb = static_cast<R>(f3_from_cache<M>(mc, kb, g0, st.iz));
         cx<R> ua, ub;
         R f3a, f3b;
         if constexpr (TW > 0) {
@@ -386,7 +475,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
           for (int k = 0; k < M; ++k) {
             const int g = g0 + k;
             if (k < mv && g % P.ndz == 0 && g / P.ndz <= P.nvz) {
-              const double f3 = static_cast<double>(f3row[k]);
+              const double f3 = f3_from_cache<M>(mc, k, g0, st.iz);
               const int64_t og = (static_cast<int64_t>(st.tlc) * a.nvz_max + (g / P.ndz - 1)) * a.ld + b;
               a.cpg[2 * og] = static_cast<double>(L.u[k].re) * f3;
               a.cpg[2 * og + 1] = static_cast<double>(L.u[k].im) * f3;
@@ -405,61 +494,73 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
 }  // namespace
 
 // Shuffle mode: one 64-lane team per wavefront, four per 256-thread block.
-template <class R, int M>
+template <class R, int M, bool DS, bool DX>
 __global__ void __launch_bounds__(256) k_march_team_w(MarchArgs a, int s0, int s1) {
   const int team = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 64>(a, b, threadIdx.x % 64, 64, nullptr, s0, s1);
+  march_team_body<R, M, 64, DS, DX>(a, b, threadIdx.x % 64, 64, nullptr, s0, s1);
 }
 
 // LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64).
-template <class R, int M>
+template <class R, int M, bool DS, bool DX>
 __global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s1) {
   extern __shared__ char smem[];
   const int team = blockIdx.x;
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 0>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<cx<R> *>(smem), s0, s1);
+  march_team_body<R, M, 0, DS, DX>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<cx<R> *>(smem), s0, s1);
 }
 
 namespace {
 
-template <class R, int M>
+template <class R, int M, bool DS, bool DX = false>
 int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
   if (lanes == 64) {
     const int grid = (a.team_nrays + 3) / 4;
-    hipLaunchKernelGGL((k_march_team_w<R, M>), dim3(grid), dim3(256), 0, st, a, s0, s1);
+    hipLaunchKernelGGL((k_march_team_w<R, M, DS, DX>), dim3(grid), dim3(256), 0, st, a, s0, s1);
   } else {
     if (lanes % 64 || lanes > 512) return -1;
     const size_t lds = 3 * static_cast<size_t>(lanes) * sizeof(cx<R>);
-    hipLaunchKernelGGL((k_march_team_l<R, M>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
+    hipLaunchKernelGGL((k_march_team_l<R, M, DS, DX>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // namespace
 
+// Rows per lane and where the lane keeps its state: complex64 holds
+// everything in VGPRs up to 12 rows, streams the Galerkin coefficients at 16,
+// 24 and 32 rows, and also the elimination arrays at 40, 48 and 64 rows;
+// complex128 likewise at up to 6, 8-12 and 16-32 rows.
+bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 16 : rows >= 8; }
+bool team_rows_elim_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 40 : rows >= 16; }
+
 bool team_shape_supported(int precision, int rows, int lanes) {
   const bool lanes_ok = lanes >= 64 && lanes <= 512 && lanes % 64 == 0;
   if (!lanes_ok) return false;
-  if (precision == CPR_C64) return rows == 4 || rows == 8 || rows == 12 || rows == 16;
-  return rows == 2 || rows == 4 || rows == 6 || rows == 8;
+  if (precision == CPR_C64)
+    return rows == 4 || rows == 8 || rows == 12 || rows == 16 || rows == 24 || rows == 32 || rows == 40 ||
+           rows == 48 || rows == 64;
+  return rows == 2 || rows == 4 || rows == 6 || rows == 8 || rows == 12 || rows == 16 || rows == 24 || rows == 32;
 }
 
 void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
-  const int cand64[] = {16, 12, 8, 4};
-  const int cand128[] = {8, 6, 4, 2};
+  const int cand64[] = {4, 8, 12, 16, 24, 32, 40, 48, 64};
+  const int cand128[] = {2, 4, 6, 8, 12, 16, 24, 32};
   const int *cand = precision == CPR_C64 ? cand64 : cand128;
+  const int ncand = precision == CPR_C64 ? 9 : 8;
   int best_r = 0, best_l = 0;
-  long best_cost = -1;
-  for (int i = 0; i < 4; ++i) {
+  double best_cost = -1;
+  for (int i = 0; i < ncand; ++i) {
     const int m = cand[i];
     int l = (nz + m - 1) / m;
     l = l <= 64 ? 64 : (l + 63) / 64 * 64;
     if (l > 512) continue;
-    // padded rows plus a reduced-system term per lane
-    const long cost = static_cast<long>(l) * m + 8L * l;
+    // padded row work, a reduced-system term per lane, and premiums for
+    // streaming coefficients and elimination arrays
+    const double prem = team_rows_elim_stream(precision, m) ? 1.6 : (team_rows_stream(precision, m) ? 1.15 : 1.0);
+    const double cost = static_cast<double>(l) * m * prem + 8.0 * l;
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
       best_r = m;
@@ -470,28 +571,58 @@ void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
   *lanes = best_l;
 }
 
+int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays) {
+  if (!team_rows_stream(precision, rows)) return 0;
+  const int64_t row_bytes = precision == CPR_C64 ? sizeof(CoefRow<float>) : sizeof(CoefRow<double>);
+  return static_cast<int64_t>(nrays) * rows * lanes * row_bytes;
+}
+
+int64_t team_xbuf_bytes(int precision, int rows, int lanes, int nrays) {
+  if (!team_rows_elim_stream(precision, rows)) return 0;
+  const int64_t cx_bytes = precision == CPR_C64 ? sizeof(cx<float>) : sizeof(cx<double>);
+  return static_cast<int64_t>(nrays) * rows * lanes * 2 * cx_bytes;
+}
+
 int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, int s1, hipStream_t st) {
   if (precision == CPR_C64) {
     switch (rows) {
       case 4:
-        return launch_rows<float, 4>(a, lanes, s0, s1, st);
+        return launch_rows<float, 4, false>(a, lanes, s0, s1, st);
       case 8:
-        return launch_rows<float, 8>(a, lanes, s0, s1, st);
+        return launch_rows<float, 8, false>(a, lanes, s0, s1, st);
       case 12:
-        return launch_rows<float, 12>(a, lanes, s0, s1, st);
+        return launch_rows<float, 12, false>(a, lanes, s0, s1, st);
       case 16:
-        return launch_rows<float, 16>(a, lanes, s0, s1, st);
+        return launch_rows<float, 16, true>(a, lanes, s0, s1, st);
+      case 24:
+        return launch_rows<float, 24, true>(a, lanes, s0, s1, st);
+      case 32:
+        return launch_rows<float, 32, true>(a, lanes, s0, s1, st);
+      case 40:
+        return launch_rows<float, 40, true, true>(a, lanes, s0, s1, st);
+      case 48:
+        return launch_rows<float, 48, true, true>(a, lanes, s0, s1, st);
+      case 64:
+        return launch_rows<float, 64, true, true>(a, lanes, s0, s1, st);
     }
   } else {
     switch (rows) {
       case 2:
-        return launch_rows<double, 2>(a, lanes, s0, s1, st);
+        return launch_rows<double, 2, false>(a, lanes, s0, s1, st);
       case 4:
-        return launch_rows<double, 4>(a, lanes, s0, s1, st);
+        return launch_rows<double, 4, false>(a, lanes, s0, s1, st);
       case 6:
-        return launch_rows<double, 6>(a, lanes, s0, s1, st);
+        return launch_rows<double, 6, false>(a, lanes, s0, s1, st);
       case 8:
-        return launch_rows<double, 8>(a, lanes, s0, s1, st);
+        return launch_rows<double, 8, true>(a, lanes, s0, s1, st);
+      case 12:
+        return launch_rows<double, 12, true>(a, lanes, s0, s1, st);
+      case 16:
+        return launch_rows<double, 16, true, true>(a, lanes, s0, s1, st);
+      case 24:
+        return launch_rows<double, 24, true, true>(a, lanes, s0, s1, st);
+      case 32:
+        return launch_rows<double, 32, true, true>(a, lanes, s0, s1, st);
     }
   }
   return -1;

@@ -208,6 +208,7 @@ class CuPyRAM:
         self._dz = kwargs.get("dz", CuPyRAM._dzf * 1500 / max_freq)
         self._ndr = int(kwargs.get("ndr", CuPyRAM._ndr_default))
         self._ndz = int(kwargs.get("ndz", CuPyRAM._ndz_default))
+        self._zmplt_given = "zmplt" in kwargs
         self._zmplt = kwargs.get("zmplt", max(float(numpy.max(e["rbzb"][:, 1])) for e in self._env))
         rmax_default = max(max(float(numpy.max(e["rp_ss"])), float(numpy.max(e["rp_sb"])),
                                float(numpy.max(e["rbzb"][:, 0]))) for e in self._env)
@@ -252,7 +253,12 @@ class CuPyRAM:
         else:
             zmax_e = numpy.full(ne, float(numpy.max(zmax_sb)) + self._lyrw * self._lambda)
         nz_e = numpy.floor(zmax_e / dz_e).astype(numpy.int64) - 1
-        nzplt_e = numpy.floor(self._zmplt / dz_e).astype(numpy.int64)
+        if self._per_ray_grid and not self._zmplt_given:
+            zmplt_e = numpy.array([float(numpy.max(e["rbzb"][:, 1])) for e in self._env])
+        else:
+            zmplt_e = self._per_env(self._zmplt, ne)
+        nzplt_e = numpy.floor(zmplt_e / dz_e).astype(numpy.int64)
+        self._zmplt_e = zmplt_e
         nvr_e = numpy.floor(rmax_e / (dr_e * self._ndr)).astype(numpy.int64)
         rmax_trunc = nvr_e * dr_e * self._ndr
         nsteps_e = numpy.round(rmax_trunc / dr_e).astype(numpy.int64) - 1
@@ -349,37 +355,88 @@ class CuPyRAM:
 
     # -- run (cupyram.py:273-380) ------------------------------------------------
 
-    def _make_plan(self, idx, stream=None):
+    def _make_plan(self, idx, stream=None, shape=None, variant=None):
         cfg = _abi.CprConfig()
         cfg.nrays = len(idx)
         cfg.precision = self._precision
-        cfg.variant = _abi.VARIANTS[self._variant]
+        cfg.variant = _abi.VARIANTS[variant or self._variant]
         cfg.compute_grids = 1 if self._compute_grids else 0
-        cfg.nvr_max = self._nvr
-        cfg.nvz_max = self._nvz if self._compute_grids else 0
-        if self._team_shape:
-            cfg.team_rows, cfg.team_lanes = self._team_shape
+        cfg.nvr_max = int(self._rays["nvr"][idx].max())
+        cfg.nvz_max = int(self._rays["nvz"][idx].max()) if self._compute_grids else 0
+        shape = shape or self._team_shape
+        if shape:
+            cfg.team_rows, cfg.team_lanes = shape
         return _abi.Plan(cfg, self._rays[idx], self._envs[idx], self._env_data, self._pade[idx], stream)
+
+    def _buckets(self):
+        """Group the calculations by kernel shape: one plan per register-team
+        shape (rows per lane x lanes per ray), one for the thread march.  A
+        batch whose rays share a grid is a single bucket, as in the reference;
+        uneven batches are no longer padded to the deepest ray."""
+        nc = self._total_batch
+        if self._variant == "thread" or self._team_shape:
+            return [(numpy.arange(nc), self._team_shape, self._variant)]
+        from .autotune import tuned_shape
+        nz = self._rays["nz"]
+        table = {}
+        shapes = []
+        for z in nz:
+            z = int(z)
+            if z not in table:
+                table[z] = tuned_shape(self._precision, z) or _abi.team_shape(self._precision, z)
+            shapes.append(table[z])
+        if self._variant == "team" and any(sh == (0, 0) for sh in shapes):
+            raise ValueError("some rays are too deep for the register-resident team march; use variant='auto'")
+        groups = {}
+        for c, sh in enumerate(shapes):
+            groups.setdefault(sh, []).append(c)
+        out = []
+        for sh, ids in sorted(groups.items(), key=lambda kv: -len(kv[1])):
+            if sh == (0, 0):
+                out.append((numpy.array(ids), None, "thread"))
+            else:
+                out.append((numpy.array(ids), sh, "team"))
+        return out
 
     def run(self):
         """March every ray; returns the reference's result dictionary."""
         t0 = time.perf_counter()
         self.setup()
         nc = self._total_batch
-        plan = self._make_plan(numpy.arange(nc))
+        buckets = self._buckets()
+        tll = numpy.zeros((nc, self._nvr))
+        cpl = numpy.zeros((nc, self._nvr), numpy.complex128)
+        tlg = numpy.zeros((nc, self._nvz, self._nvr)) if self._compute_grids else None
+        cpg = numpy.zeros((nc, self._nvz, self._nvr), numpy.complex128) if self._compute_grids else None
+        plans = []
         try:
-            plan.selfs()
-            nmax = int(self._rays["nsteps"].max())
-            if self._steps_per_launch > 0:
-                plan.march_graph(self._steps_per_launch)
-            else:
-                plan.march(0, nmax)
-            plan.sync()
-            self.march_ms, self.march_launches = plan.march_time()
-            tll, cpl, tlg, cpg = plan.read_outputs(grids=self._compute_grids)
-            self.plan_variant = plan.describe(0)
+            for idx, shape, variant in buckets:
+                plans.append((idx, self._make_plan(idx, shape=shape, variant=variant)))
+            for _, plan in plans:
+                plan.selfs()
+            for idx, plan in plans:
+                nmax = int(self._rays["nsteps"][idx].max())
+                if self._steps_per_launch > 0:
+                    plan.march_graph(self._steps_per_launch)
+                else:
+                    plan.march(0, nmax)
+            self.march_ms, self.march_launches = 0.0, 0
+            self.plan_variants = []
+            for idx, plan in plans:
+                plan.sync()
+                ms, n = plan.march_time()
+                self.march_ms = max(self.march_ms, ms)
+                self.march_launches += n
+                a, b, g, h = plan.read_outputs(grids=self._compute_grids)
+                nv = a.shape[1]
+                tll[idx, :nv], cpl[idx, :nv] = a, b
+                if self._compute_grids:
+                    tlg[idx, :g.shape[1], :nv], cpg[idx, :g.shape[1], :nv] = g, h
+                self.plan_variants.append((len(idx), plan.describe(0)))
+            self.plan_variant = self.plan_variants[0][1]
         finally:
-            plan.close()
+            for _, plan in plans:
+                plan.close()
         self.proc_time = time.perf_counter() - t0
         if self._benchmark_propagation:
             self.propagation_time = self.march_ms / 1e3

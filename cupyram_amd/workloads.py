@@ -8,7 +8,7 @@ munk_batch      10,000 rays, Munk profile (axis 1300 m, eps 0.00737) 5000 m
 uneven_batch    50,000 rays, depth log-uniform 100-5000 m, freq log-uniform
                 25-500 Hz, dz = lambda/10, dr = lambda/2, rmax 20 km, np 6,
                 Munk profile scaled to the depth plus 32 knots of N(0, 5^2)
-                m/s (seed 3).
+                m/s, zs 25 m, zr 50 m (seed 3).
 wedge_queries   upslope wedge 200 -> 20 m over 10 km, sediment layer 5-30 m
                 over basement, freq log-uniform 50-1000 Hz (seed 5).
 pekeris_c1      the single CPU-runnable Pekeris ray (100 m of 1500 m/s water
@@ -58,7 +58,7 @@ def uneven_batch(nrays=50000, seed=3, rmax=20000.0, np_pade=6):
     cb = rng.uniform(1550.0, 1900.0, size=nrays)
     rho = rng.uniform(1.3, 2.0, size=nrays)
     att = rng.uniform(0.1, 1.0, size=nrays)
-    return dict(freq=freq, freq_per_ray=True, zs=np.minimum(0.3 * depth, 100.0).min(), zr=None,
+    return dict(freq=freq, freq_per_ray=True, zs=25.0, zr=50.0,
                 z_ss=z_ss, rp_ss=np.array([0.0]), cw=cw, z_sb=np.array([0.0]), rp_sb=np.array([0.0]),
                 cb=[np.array([[v]]) for v in cb], rhob=[np.array([[v]]) for v in rho],
                 attn=[np.array([[v]]) for v in att],

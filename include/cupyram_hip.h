@@ -114,6 +114,12 @@ int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double 
 int cpr_read_field(cpr_plan *p, double *u, int nz_max);
 /* Bytes of device memory held by the plan. */
 int64_t cpr_plan_device_bytes(cpr_plan *p);
+/* Team shape (rows per lane, lanes per ray) the library would pick for a
+ * column of nz rows; rows = lanes = 0 when no register-resident team holds
+ * it (such rays use the thread march).  Used by the host to bucket uneven
+ * batches (no counterpart in the reference, which pads every ray to the
+ * batch maximum, cupyram/cupyram.py:596-597). */
+int cpr_team_shape(int precision, int nz, int *rows, int *lanes);
 /* Kernel variant and team shape chosen for ray b (for tests/autotuning). */
 int cpr_plan_describe(cpr_plan *p, int b, int *variant, int *team_rows, int *team_lanes);
 

@@ -102,7 +102,7 @@ def oracle_for(model, env_index=0, freq_index=0):
                        dr=float(ray["dr"]), dz=float(ray["dz"]), zmax=float(ray["zmax"]),
                        rmax=float(np.asarray(model._rmax_input).ravel()[env_index]
                                   if np.ndim(model._rmax_input) else model._rmax_input),
-                       zmplt=float(model._zmplt), rs=float(ray["rs"]), np_pade=model._np, ns=model._ns,
+                       zmplt=float(model._zmplt_e[env_index]), rs=float(ray["rs"]), np_pade=model._np, ns=model._ns,
                        ndr=model._ndr, ndz=model._ndz, lyrw=float(model._lyrw))
 
 

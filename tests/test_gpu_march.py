@@ -132,3 +132,60 @@ def test_multi_frequency_matches_single(require_gpu):
         assert d < 0.1
         if f == max(freqs):
             assert d < 1e-9
+
+
+@pytest.mark.parametrize("precision,shape", [("c64", (4, 128)), ("c64", (8, 64)), ("c64", (12, 64)),
+                                             ("c64", (16, 64)), ("c64", (24, 64)), ("c64", (32, 64)),
+                                             ("c64", (8, 128)), ("c64", (16, 128)),
+                                             ("c128", (2, 192)), ("c128", (4, 128)), ("c128", (6, 64)),
+                                             ("c128", (8, 64)), ("c128", (12, 64)), ("c128", (4, 192)),
+                                             ("c64", (40, 64)), ("c64", (48, 64)), ("c64", (64, 64)),
+                                             ("c128", (16, 64)), ("c128", (24, 64)), ("c128", (32, 64))])
+def test_every_team_shape_matches_oracle(require_gpu, precision, shape):
+    """Register rows (c64 4..12, c128 2..6), streamed-coefficient rows
+    (c64 16..32, c128 8..12) and streamed elimination arrays (c64 40..64,
+    c128 16..32), wave-shuffle (64 lanes) and LDS (>64 lanes) teams, on the
+    wedge (sloping bathymetry moves the water/bottom split)."""
+    tol = 0.1 if precision == "c128" else 0.5
+    worst = _compare_with_oracle(envs.wedge(), precision, "team", tol, grids=True, team_shape=shape)
+    if precision == "c128":
+        assert worst < 1e-4
+
+
+def _uneven(n=6, rmax=3000.0, seed=11):
+    from cupyram_amd import workloads
+    kw = workloads.uneven_batch(nrays=n, seed=seed, rmax=rmax)
+    return kw
+
+
+@pytest.mark.parametrize("precision", ["c64", "c128"])
+def test_uneven_batch_per_ray_grids(require_gpu, precision):
+    """Per-ray frequency, dz = lambda/10 and dr = lambda/2: rays land in
+    different kernel buckets and each matches its own oracle run."""
+    kw = _uneven()
+    m, res = _run(kw, precision=precision)
+    assert len(m.plan_variants) >= 2
+    tl = res["TL Line"]
+    for e in range(m._n_env):
+        o = O.run(envs.oracle_for(m, e, 0), grids=False)
+        s = envs.tl_stats(tl[e, :o["TL Line"].size], o["TL Line"])
+        assert s["rms"] <= (0.1 if precision == "c128" else 0.5), (e, s)
+        assert np.all(np.isnan(tl[e, o["TL Line"].size:]))
+
+
+def test_bucketing_independent_of_batch_composition(require_gpu):
+    """A ray's result does not depend on which other rays share its batch."""
+    kw = _uneven(n=6)
+    _, full = _run(kw, precision="c64")
+    sub = dict(kw)
+    keep = [1, 4]
+    for k in ("z_ss", "cw", "cb", "rhob", "attn", "rbzb"):
+        sub[k] = [kw[k][i] for i in keep]
+    for k in ("freq", "dz", "dr"):
+        sub[k] = np.asarray(kw[k])[keep]
+    sub["batch_size"] = 2
+    _, part = _run(sub, precision="c64")
+    for j, i in enumerate(keep):
+        a, b = full["TL Line"][i], part["TL Line"][j]
+        n = np.sum(np.isfinite(b))
+        assert np.array_equal(a[:n], b[:n])
