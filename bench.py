@@ -162,11 +162,59 @@ def bench_uneven(args):
     return 0
 
 
+def bench_wedge(args):
+    """Surrogate-first answering of the wedge queries with complex64 PE
+    fallback, beside the plain complex64 PE march of every query; TL error of
+    both against the complex128 march on the domain grid (TL clipped to
+    20-140 dB)."""
+    import torch
+    from cupyram_amd.surrogate import dataset as D
+    from cupyram_amd.surrogate.router import Router
+    from cupyram_amd.surrogate.train import field_rms, load_ensemble, pe_fields, summary
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+    dom = D.WEDGE
+    n = args.rays if args.rays != 10000 else 100000
+    q = D.wedge_queries(n, 5)
+    members = load_ensemble(dom.name)
+    log(f"wedge: {n} queries, {len(members)} ensemble members")
+    router = Router(dom, members, threshold_db=3.0, spot_fraction=0.01, seed=5, precision="c64", log=None)
+    router.ensemble(q.take(np.arange(min(n, 256))))          # warm-up (buffers, kernels)
+    torch.cuda.synchronize()
+    res = router.run(q)
+    log(f"wedge: routed {res.frac_routed:.3f} in {res.seconds['total']:.1f} s")
+    t0 = time.perf_counter()
+    tl64, t64, cells = pe_fields(dom, q, "c64", chunk=20000, log=log)
+    torch.cuda.synchronize()
+    wall64 = time.perf_counter() - t0
+    tl128, t128, _ = pe_fields(dom, q, "c128", chunk=20000, log=log)
+    e2e = summary(field_rms(dom, res.tl, tl128))
+    pe_err = summary(field_rms(dom, tl64, tl128))
+    acc = ~res.routed
+    sur = summary(field_rms(dom, res.tl[torch.as_tensor(np.nonzero(acc)[0], device="cuda")],
+                            tl128[torch.as_tensor(np.nonzero(acc)[0], device="cuda")])) if acc.any() else None
+    line = {"metric": "wedge queries/s, surrogate-first with complex64 PE fallback (TL error vs complex128)",
+            "value": n / res.seconds["total"], "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+            "data": "synthetic (seeded wedge queries, seed 5)",
+            "config": {"workload": f"wedge_queries: {n} queries, 200->20 m over 10 km, sediment 5-30 m over "
+                                   "basement, freq logU 50-1000 Hz, np 6, output 200 x 64",
+                       "members": len(members), "threshold_db": 3.0, "spot_fraction": 0.01},
+            "frac_routed_to_pe": res.frac_routed, "batch_fallback": res.batch_fallback,
+            "seconds": res.seconds, "ensemble_spread_median_db": float(np.median(res.spread_db)),
+            "spot_checked": int(len(res.spot)),
+            "spot_err_rms_db": float(np.sqrt(np.mean(res.spot_err_db ** 2))) if len(res.spot) else None,
+            "tl_err_end_to_end_vs_c128": e2e, "tl_err_accepted_surrogate_vs_c128": sur,
+            "pe_only_c64": {"queries_per_s": n / wall64, "march_s": t64, "wall_s": wall64, "cell_steps": cells,
+                            "tl_err_vs_c128": pe_err},
+            "c128_march_s": t128}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--rays", type=int, default=10000)
     ap.add_argument("--variant", default="auto")
     ap.add_argument("--launch-steps", type=int, default=100, help="range steps per kernel launch")
@@ -175,7 +223,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--cpu-rmax", type=float, default=5000.0)
-    ap.add_argument("--workload", default="munk", choices=["munk", "uneven"])
+    ap.add_argument("--workload", default="munk", choices=["munk", "uneven", "wedge"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (torch) so that HBM counters can be scaled")
     args = ap.parse_args(argv)
@@ -190,6 +238,8 @@ def main(argv=None):
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload == "wedge":
+        return bench_wedge(args)
     if args.workload == "uneven":
         if args.rays == 10000:
             args.rays = 50000

@@ -84,6 +84,25 @@ _SYMBOLS = {
     "cpr_team_shape": ([C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "cpr_plan_describe": ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
                           C.c_int),
+    # surrogate kernels (device pointers as integers)
+    "cpr_fno_gemm": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                      C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                      C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                      C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_spec_analysis": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                               C.c_int, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_spec_synth": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                            C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_gelu_bwd": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "cpr_fno_spec_expand": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_spec_fold": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_colsum": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_mse": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
+                     C.c_void_p, C.c_void_p], C.c_int),
+    "cpr_fno_adam": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
+                      C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_void_p], C.c_int),
+    "cpr_fno_fill": ([C.c_void_p, C.c_int64, C.c_float, C.c_void_p], C.c_int),
+    "cpr_fno_cast_bf16": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
 }
 
 

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcupyram_hip.so")
-SOURCES = ["cpr_abi.hip", "march_thread.hip", "march_team.hip", "pade_host.cpp"]
+SOURCES = ["cpr_abi.hip", "march_thread.hip", "march_team.hip", "fno_kernels.hip", "pade_host.cpp"]
 ARCH = os.environ.get("CPR_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -41,6 +41,8 @@ int fail(const std::string &m) {
 
 }  // namespace
 
+int cpr_fail(const std::string &m) { return fail(m); }
+
 struct cpr_plan {
   cpr_config cfg{};
   int nrays = 0, np_ = 0, nz_max = 0, max_steps = 0;

@@ -3,6 +3,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <string>
 
 #include "../../include/cupyram_hip.h"
 
@@ -11,6 +12,9 @@
 #else
 #define CPR_HD
 #endif
+
+// Records m as the thread's last error (cpr_last_error) and returns -1.
+int cpr_fail(const std::string &m);
 
 int cpr_pade_impl(double freq, double c0, int np_, int ns, double dr, int ip, double *pd1, double *pd2);
 int cpr_pade_batch_impl(int n, const double *freq, const double *c0, int np_, int ns, const double *dr, int ip,

@@ -1,0 +1,483 @@
+// Kernels of the learned TL surrogate (a 1-D Fourier-operator network over
+// depth), written for gfx950: BF16 operands, FP32 accumulation on MFMA.
+//
+//   k_gemm          C = alpha * op(A) op(B) [+ bias] [+ addend] [GELU], batched,
+//                   optional split-K with FP32 atomics; 128x128 block tiles,
+//                   four waves of 64x64, v_mfma_f32_32x32x16_bf16
+//   k_spec_analysis modes[k][s][c] = sum_z Bc[k][z] x[s][z][c] (and Bs -> imag)
+//   k_spec_synth    y[s][z][c] (+)= sum_k Bc[k][z] m[k][s][c] + Bs[k][z] m[k][s][C+c]
+//   k_gelu_bwd, k_colsum, k_mse, k_adam, k_fill, k_cast
+//
+// The surrogate has no counterpart in the reference; it is trained only on TL
+// fields produced by this repository's march.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <string>
+
+#include "cpr_internal.h"
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 32, LDP = BK + 8;
+
+__device__ inline float gelu_f(float x) {
+  // tanh form of GELU (Hendrycks & Gimpel 2016)
+  const float k = 0.7978845608028654f;
+  const float u = k * (x + 0.044715f * x * x * x);
+  return 0.5f * x * (1.0f + tanhf(u));
+}
+__device__ inline float gelu_grad(float x) {
+  const float k = 0.7978845608028654f;
+  const float x2 = x * x;
+  const float u = k * (x + 0.044715f * x2 * x);
+  const float t = tanhf(u);
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * x2);
+}
+
+struct GemmArgs {
+  int M, N, K;
+  const bf16 *A;
+  int64_t lda, sA;  // batch stride
+  const bf16 *B;
+  int64_t ldb, sB;
+  void *C;
+  int64_t ldc, sC;
+  const float *bias;    // [N] or null
+  const float *addend;  // same layout as C (fp32) or null
+  int64_t ldd, sD;
+  float *pre;           // fp32 pre-activation out (GELU epilogue) or null
+  float alpha;
+  int splits;           // split-K factor (atomic epilogue when > 1)
+  int kchunk;
+};
+
+enum { EPI_F32 = 0, EPI_BF16 = 1, EPI_GELU_BF16 = 2, EPI_ATOMIC = 3 };
+
+}  // namespace
+
+template <bool TA, bool TB, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ bf16 As[BM][LDP];
+  __shared__ bf16 Bs[BN][LDP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int z = blockIdx.z;
+  const int batch = z / g.splits, split = z % g.splits;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const bf16 *A = g.A + batch * g.sA;
+  const bf16 *B = g.B + batch * g.sB;
+  const int kbeg = split * g.kchunk;
+  int kend = kbeg + g.kchunk;
+  kend = kend < g.K ? kend : g.K;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  const bf16 zero = static_cast<bf16>(0.0f);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    // stage op(A)[m0..m0+127][k0..k0+31] into As[m][k]
+    if (!TA) {
+      const int r = tid >> 1, kk = (tid & 1) * 16;
+      const int m = m0 + r;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int k = k0 + kk + i;
+        As[r][kk + i] = (m < g.M && k < kend) ? A[static_cast<int64_t>(m) * g.lda + k] : zero;
+      }
+    } else {
+      const int kk = tid >> 3, mm = (tid & 7) * 16;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int m = m0 + mm + i;
+        As[mm + i][kk] = (m < g.M && k < kend) ? A[static_cast<int64_t>(k) * g.lda + m] : zero;
+      }
+    }
+    // stage op(B)[k0..k0+31][n0..n0+127] into Bs[n][k]
+    if (!TB) {
+      const int kk = tid >> 3, nn = (tid & 7) * 16;
+      const int k = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int n = n0 + nn + i;
+        Bs[nn + i][kk] = (n < g.N && k < kend) ? B[static_cast<int64_t>(k) * g.ldb + n] : zero;
+      }
+    } else {
+      const int r = tid >> 1, kk = (tid & 1) * 16;
+      const int n = n0 + r;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int k = k0 + kk + i;
+        Bs[r][kk + i] = (n < g.N && k < kend) ? B[static_cast<int64_t>(n) * g.ldb + k] : zero;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      bf16x8 af[2], bfr[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const bf16 *pa = &As[wm + i * 32 + (lane & 31)][ks + 8 * (lane >> 5)];
+        const bf16 *pb = &Bs[wn + i * 32 + (lane & 31)][ks + 8 * (lane >> 5)];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          af[i][e] = pa[e];
+          bfr[i][e] = pb[e];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // epilogue: lane holds column n = lane%32 and rows 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int n = n0 + wn + j * 32 + (lane & 31);
+        if (m >= g.M || n >= g.N) continue;
+        float v = g.alpha * acc[i][j][r];
+        const int64_t oc = batch * g.sC + static_cast<int64_t>(m) * g.ldc + n;
+        if (EPI == EPI_ATOMIC) {
+          if (split == 0 && g.bias) v += g.bias[n];
+          atomicAdd(static_cast<float *>(g.C) + oc, v);
+          continue;
+        }
+        if (g.bias) v += g.bias[n];
+        if (g.addend) v += g.addend[batch * g.sD + static_cast<int64_t>(m) * g.ldd + n];
+        if (EPI == EPI_F32) {
+          static_cast<float *>(g.C)[oc] = v;
+        } else if (EPI == EPI_BF16) {
+          static_cast<bf16 *>(g.C)[oc] = static_cast<bf16>(v);
+        } else {
+          if (g.pre) g.pre[batch * g.sC + static_cast<int64_t>(m) * g.ldc + n] = v;
+          static_cast<bf16 *>(g.C)[oc] = static_cast<bf16>(gelu_f(v));
+        }
+      }
+}
+
+// modes: out[k][s][c] = sum_z bc[k][z] x[s][z][c], out[k][s][C+c] = sum_z bs[k][z] x[s][z][c]
+// x is [S][Z][C] (T = bf16 or float), out [K][S][2C] bf16.  One block of C
+// threads per sample; C <= 256, Z <= 64, K <= 32.
+template <class T>
+__global__ void k_spec_analysis(const T *x, bf16 *out, const float *bc, const float *bs, int S, int Z, int C, int K) {
+  __shared__ float sb[2][32][64];
+  const int s = blockIdx.x, c = threadIdx.x;
+  for (int i = threadIdx.x; i < K * Z; i += blockDim.x) {
+    sb[0][i / Z][i % Z] = bc[i];
+    sb[1][i / Z][i % Z] = bs[i];
+  }
+  __syncthreads();
+  if (s >= S || c >= C) return;
+  float re[32], im[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) re[k] = im[k] = 0.0f;
+  for (int zz = 0; zz < Z; ++zz) {
+    const float v = static_cast<float>(x[(static_cast<int64_t>(s) * Z + zz) * C + c]);
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      if (k < K) {
+        re[k] += sb[0][k][zz] * v;
+        im[k] += sb[1][k][zz] * v;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (k < K) {
+      const int64_t o = (static_cast<int64_t>(k) * S + s) * 2 * C;
+      out[o + c] = static_cast<bf16>(re[k]);
+      out[o + C + c] = static_cast<bf16>(im[k]);
+    }
+  }
+}
+
+// y[s][z][c] (+)= sum_k bc[k][z] m[k][s][c] + bs[k][z] m[k][s][C+c]; y fp32.
+__global__ void k_spec_synth(const bf16 *m, float *y, const float *bc, const float *bs, int S, int Z, int C, int K,
+                             int accumulate) {
+  __shared__ float sb[2][32][64];
+  const int s = blockIdx.x, c = threadIdx.x;
+  for (int i = threadIdx.x; i < K * Z; i += blockDim.x) {
+    sb[0][i / Z][i % Z] = bc[i];
+    sb[1][i / Z][i % Z] = bs[i];
+  }
+  __syncthreads();
+  if (s >= S || c >= C) return;
+  float re[32], im[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (k < K) {
+      const int64_t o = (static_cast<int64_t>(k) * S + s) * 2 * C;
+      re[k] = static_cast<float>(m[o + c]);
+      im[k] = static_cast<float>(m[o + C + c]);
+    } else {
+      re[k] = im[k] = 0.0f;
+    }
+  }
+  for (int zz = 0; zz < Z; ++zz) {
+    float v = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+      if (k < K) v += sb[0][k][zz] * re[k] + sb[1][k][zz] * im[k];
+    const int64_t o = (static_cast<int64_t>(s) * Z + zz) * C + c;
+    y[o] = accumulate ? y[o] + v : v;
+  }
+}
+
+// dx = dy * gelu'(pre) (pre = null: identity), written as bf16 for the next GEMMs
+__global__ void k_gelu_bwd(const float *dy, const float *pre, bf16 *dx, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dx[i] = static_cast<bf16>(pre ? dy[i] * gelu_grad(pre[i]) : dy[i]);
+}
+
+// Spectral weights: complex W[k] = Wr[k] + i Wi[k] ([K][C][C], row = input
+// channel) acts on the real row vector [Re | Im] as the block matrix
+// [[Wr, Wi], [-Wi, Wr]] ([K][2C][2C]).  expand writes the block (bf16);
+// fold maps a block gradient G to (dWr, dWi) = (G00 + G11, G01 - G10).
+__global__ void k_spec_expand(const float *wr, const float *wi, bf16 *blk, int K, int C) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = static_cast<int64_t>(K) * C * C;
+  if (i >= n) return;
+  const int k = static_cast<int>(i / (C * C)), r = static_cast<int>((i / C) % C), c = static_cast<int>(i % C);
+  bf16 *b = blk + static_cast<int64_t>(k) * 4 * C * C;
+  const int L = 2 * C;
+  b[r * L + c] = static_cast<bf16>(wr[i]);
+  b[r * L + C + c] = static_cast<bf16>(wi[i]);
+  b[(C + r) * L + c] = static_cast<bf16>(-wi[i]);
+  b[(C + r) * L + C + c] = static_cast<bf16>(wr[i]);
+}
+
+__global__ void k_spec_fold(const float *g, float *gr, float *gi, int K, int C) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = static_cast<int64_t>(K) * C * C;
+  if (i >= n) return;
+  const int k = static_cast<int>(i / (C * C)), r = static_cast<int>((i / C) % C), c = static_cast<int>(i % C);
+  const float *b = g + static_cast<int64_t>(k) * 4 * C * C;
+  const int L = 2 * C;
+  gr[i] = b[r * L + c] + b[(C + r) * L + C + c];
+  gi[i] = b[r * L + C + c] - b[(C + r) * L + c];
+}
+
+// out[n] += sum_m x[m][n] (column sums of an M x N bf16 matrix)
+__global__ void k_colsum(const bf16 *x, float *out, int64_t M, int N, int rows_per_block) {
+  const int n = threadIdx.x + blockIdx.y * blockDim.x;
+  if (n >= N) return;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+  int64_t r1 = r0 + rows_per_block;
+  r1 = r1 < M ? r1 : M;
+  float s = 0.0f;
+  for (int64_t r = r0; r < r1; ++r) s += static_cast<float>(x[r * N + n]);
+  atomicAdd(out + n, s);
+}
+
+// Masked mean squared error on column 0 of pred [P][ldp]; grad written to
+// dpred (bf16, same layout, other columns zero); loss_sum[0] += sum of squares,
+// loss_sum[1] += count.  scale = 2 / (number of valid points).
+__global__ void k_mse(const float *pred, int ldp, const float *target, const float *mask, bf16 *dpred, int64_t P,
+                      float scale, float *loss_sum) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const float w = mask[p];
+  const float d = (pred[p * ldp] - target[p]) * w;
+  for (int c = 0; c < ldp; ++c) dpred[p * ldp + c] = static_cast<bf16>(c == 0 ? scale * d : 0.0f);
+  atomicAdd(loss_sum, d * d);
+  atomicAdd(loss_sum + 1, w);
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the BF16 copy.
+__global__ void k_adam(float *w, const float *g, float *m1, float *m2, bf16 *wb, int64_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, float wd) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i] + wd * w[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  const float wn = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  w[i] = wn;
+  if (wb) wb[i] = static_cast<bf16>(wn);
+}
+
+__global__ void k_fill(float *x, int64_t n, float v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void k_cast_bf16(const float *x, bf16 *y, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = static_cast<bf16>(x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI (declared in include/cupyram_hip.h)
+// ---------------------------------------------------------------------------
+namespace {
+inline unsigned blocks_for(int64_t n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+int launched(const char *name) {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : cpr_fail(std::string(name) + ": " + hipGetErrorString(e));
+}
+
+template <bool TA, bool TB>
+int launch_gemm_epi(const GemmArgs &g, int epi, int batch, hipStream_t st) {
+  const dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, batch * g.splits);
+  switch (epi) {
+    case EPI_F32:
+      hipLaunchKernelGGL((k_gemm<TA, TB, EPI_F32>), grid, dim3(256), 0, st, g);
+      break;
+    case EPI_BF16:
+      hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BF16>), grid, dim3(256), 0, st, g);
+      break;
+    case EPI_GELU_BF16:
+      hipLaunchKernelGGL((k_gemm<TA, TB, EPI_GELU_BF16>), grid, dim3(256), 0, st, g);
+      break;
+    case EPI_ATOMIC:
+      hipLaunchKernelGGL((k_gemm<TA, TB, EPI_ATOMIC>), grid, dim3(256), 0, st, g);
+      break;
+    default:
+      return cpr_fail("cpr_fno_gemm: unknown epilogue");
+  }
+  return launched("cpr_fno_gemm");
+}
+}  // namespace
+
+extern "C" int cpr_fno_gemm(int ta, int tb, int epi, int batch, int M, int N, int K, float alpha, const void *A,
+                            int64_t lda, int64_t sA, const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc,
+                            int64_t sC, const float *bias, const float *addend, int64_t ldd, int64_t sD, float *pre,
+                            int splits, void *stream) {
+  if (M <= 0 || N <= 0 || K <= 0 || batch <= 0 || splits <= 0 || !A || !B || !C) return cpr_fail("cpr_fno_gemm: invalid arguments");
+  if (splits > 1 && epi != EPI_ATOMIC) return cpr_fail("cpr_fno_gemm: invalid arguments");
+  GemmArgs g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = static_cast<const bf16 *>(A);
+  g.lda = lda;
+  g.sA = sA;
+  g.B = static_cast<const bf16 *>(B);
+  g.ldb = ldb;
+  g.sB = sB;
+  g.C = C;
+  g.ldc = ldc;
+  g.sC = sC;
+  g.bias = bias;
+  g.addend = addend;
+  g.ldd = ldd;
+  g.sD = sD;
+  g.pre = pre;
+  g.alpha = alpha;
+  g.splits = splits;
+  g.kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!ta && !tb) return launch_gemm_epi<false, false>(g, epi, batch, st);
+  if (!ta && tb) return launch_gemm_epi<false, true>(g, epi, batch, st);
+  if (ta && !tb) return launch_gemm_epi<true, false>(g, epi, batch, st);
+  return launch_gemm_epi<true, true>(g, epi, batch, st);
+}
+
+extern "C" int cpr_fno_spec_analysis(const void *x, int x_is_f32, void *out, const float *bc, const float *bs, int S,
+                                     int Z, int C, int K, void *stream) {
+  if (S <= 0 || Z <= 0 || Z > 64 || C <= 0 || C > 256 || K <= 0 || K > 32) return cpr_fail("cpr_fno_spec_analysis: invalid arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (x_is_f32)
+    hipLaunchKernelGGL(k_spec_analysis<float>, dim3(S), dim3(C), 0, st, static_cast<const float *>(x),
+                       static_cast<bf16 *>(out), bc, bs, S, Z, C, K);
+  else
+    hipLaunchKernelGGL(k_spec_analysis<bf16>, dim3(S), dim3(C), 0, st, static_cast<const bf16 *>(x),
+                       static_cast<bf16 *>(out), bc, bs, S, Z, C, K);
+  return launched("cpr_fno_spec_analysis");
+}
+
+extern "C" int cpr_fno_spec_synth(const void *m, float *y, const float *bc, const float *bs, int S, int Z, int C,
+                                  int K, int accumulate, void *stream) {
+  if (S <= 0 || Z <= 0 || Z > 64 || C <= 0 || C > 256 || K <= 0 || K > 32) return cpr_fail("cpr_fno_spec_synth: invalid arguments");
+  hipLaunchKernelGGL(k_spec_synth, dim3(S), dim3(C), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const bf16 *>(m), y, bc, bs, S, Z, C, K, accumulate);
+  return launched("cpr_fno_spec_synth");
+}
+
+extern "C" int cpr_fno_gelu_bwd(const float *dy, const float *pre, void *dx, int64_t n, void *stream) {
+  if (n <= 0 || !dy || !dx) return cpr_fail("cpr_fno_gelu_bwd: invalid arguments");
+  hipLaunchKernelGGL(k_gelu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), dy, pre,
+                     static_cast<bf16 *>(dx), n);
+  return launched("cpr_fno_gelu_bwd");
+}
+
+extern "C" int cpr_fno_spec_expand(const float *wr, const float *wi, void *blk, int K, int C, void *stream) {
+  if (K <= 0 || C <= 0) return cpr_fail("cpr_fno_spec_expand: invalid arguments");
+  const int64_t n = static_cast<int64_t>(K) * C * C;
+  hipLaunchKernelGGL(k_spec_expand, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), wr, wi,
+                     static_cast<bf16 *>(blk), K, C);
+  return launched("cpr_fno_spec_expand");
+}
+
+extern "C" int cpr_fno_spec_fold(const float *g, float *gr, float *gi, int K, int C, void *stream) {
+  if (K <= 0 || C <= 0) return cpr_fail("cpr_fno_spec_fold: invalid arguments");
+  const int64_t n = static_cast<int64_t>(K) * C * C;
+  hipLaunchKernelGGL(k_spec_fold, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), g, gr,
+                     gi, K, C);
+  return launched("cpr_fno_spec_fold");
+}
+
+extern "C" int cpr_fno_colsum(const void *x, float *out, int64_t M, int N, void *stream) {
+  if (M <= 0 || N <= 0) return cpr_fail("cpr_fno_colsum: invalid arguments");
+  const int rows = 256;
+  const dim3 grid(blocks_for(M, rows), (N + 255) / 256);
+  hipLaunchKernelGGL(k_colsum, grid, dim3(N < 256 ? N : 256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const bf16 *>(x), out, M, N, rows);
+  return launched("cpr_fno_colsum");
+}
+
+extern "C" int cpr_fno_mse(const float *pred, int ldp, const float *target, const float *mask, void *dpred,
+                           int64_t P, float scale, float *loss_sum, void *stream) {
+  if (P <= 0 || ldp <= 0) return cpr_fail("cpr_fno_mse: invalid arguments");
+  hipLaunchKernelGGL(k_mse, dim3(blocks_for(P, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), pred, ldp,
+                     target, mask, static_cast<bf16 *>(dpred), P, scale, loss_sum);
+  return launched("cpr_fno_mse");
+}
+
+extern "C" int cpr_fno_adam(float *w, const float *g, float *m1, float *m2, void *wb, int64_t n, float lr, float b1,
+                            float b2, float eps, int step, float wd, void *stream) {
+  if (n <= 0 || step < 1) return cpr_fail("cpr_fno_adam: invalid arguments");
+  float c1 = 1.0f, c2 = 1.0f, p1 = 1.0f, p2 = 1.0f;
+  for (int i = 0; i < step && i < 100000; ++i) {
+    p1 *= b1;
+    p2 *= b2;
+  }
+  c1 = 1.0f - p1;
+  c2 = 1.0f - p2;
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), w, g, m1, m2,
+                     static_cast<bf16 *>(wb), n, lr, b1, b2, eps, c1, c2, wd);
+  return launched("cpr_fno_adam");
+}
+
+extern "C" int cpr_fno_fill(float *x, int64_t n, float v, void *stream) {
+  if (n <= 0) return cpr_fail("cpr_fno_fill: invalid arguments");
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x, n, v);
+  return launched("cpr_fno_fill");
+}
+
+extern "C" int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *stream) {
+  if (n <= 0) return cpr_fail("cpr_fno_cast_bf16: invalid arguments");
+  hipLaunchKernelGGL(k_cast_bf16, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x,
+                     static_cast<bf16 *>(y), n);
+  return launched("cpr_fno_cast_bf16");
+}

@@ -132,7 +132,10 @@ class CuPyRAM:
             raise ValueError("frequencies must be positive")
         self._n_env = self._batch_size
         self._total_batch = self._n_env * self._n_freq
-        self._zs, self._zr = float(zs), float(zr)
+        # zs may be one value per ray (extension, like the per-ray dz/dr arrays)
+        self._zs_e = self._per_env(zs, self._batch_size)
+        self._zs = float(zs) if numpy.ndim(zs) == 0 else self._zs_e
+        self._zr = float(zr)
         z_ss = self._normalize_to_batch(z_ss, self._batch_size, "z_ss")
         rp_ss = self._normalize_to_batch(rp_ss, self._batch_size, "rp_ss")
         cw = self._normalize_to_batch(cw, self._batch_size, "cw")
@@ -166,8 +169,8 @@ class CuPyRAM:
                     a = a[:, None]
                 profs.append(self._get_valid_slice(a)[0])
             bt, _ = self._get_valid_slice(numpy.asarray(rbzb[b], numpy.float64).reshape(-1, 2))
-            if not (zss[0] <= self._zs <= zss[-1]):
-                raise ValueError(f"Ray {b}: Source depth {self._zs}m outside range [{zss[0]}, {zss[-1]}]m")
+            if not (zss[0] <= self._zs_e[b] <= zss[-1]):
+                raise ValueError(f"Ray {b}: Source depth {self._zs_e[b]}m outside range [{zss[0]}, {zss[-1]}]m")
             if not (zss[0] <= self._zr <= zss[-1]):
                 raise ValueError(f"Ray {b}: Receiver depth {self._zr}m outside range [{zss[0]}, {zss[-1]}]m")
             if cwb.shape != (zss.size, rss.size):
@@ -203,11 +206,16 @@ class CuPyRAM:
             self._c0 = float(numpy.mean(self._c0_array))
         max_freq = float(numpy.max(self._freqs))
         self._lambda = self._c0 / max_freq
-        self._per_ray_grid = any(numpy.ndim(kwargs.get(k, 0.0)) > 0 for k in ("dz", "dr", "rmax"))
+        self._per_ray_grid = any(numpy.ndim(kwargs.get(k, 0.0)) > 0 for k in ("dz", "dr", "rmax", "ndr", "ndz"))
         self._dr = kwargs.get("dr", self._np * 1500 / max_freq)
         self._dz = kwargs.get("dz", CuPyRAM._dzf * 1500 / max_freq)
-        self._ndr = int(kwargs.get("ndr", CuPyRAM._ndr_default))
-        self._ndz = int(kwargs.get("ndz", CuPyRAM._ndz_default))
+        # output decimation; per-ray arrays are an extension of the reference's scalars
+        self._ndr_e = self._per_env(kwargs.get("ndr", CuPyRAM._ndr_default), self._batch_size).astype(numpy.int64)
+        self._ndz_e = self._per_env(kwargs.get("ndz", CuPyRAM._ndz_default), self._batch_size).astype(numpy.int64)
+        if numpy.any(self._ndr_e < 1) or numpy.any(self._ndz_e < 1):
+            raise ValueError("ndr and ndz must be >= 1")
+        self._ndr = int(self._ndr_e[0]) if numpy.ndim(kwargs.get("ndr", 1)) == 0 else self._ndr_e
+        self._ndz = int(self._ndz_e[0]) if numpy.ndim(kwargs.get("ndz", 1)) == 0 else self._ndz_e
         self._zmplt_given = "zmplt" in kwargs
         self._zmplt = kwargs.get("zmplt", max(float(numpy.max(e["rbzb"][:, 1])) for e in self._env))
         rmax_default = max(max(float(numpy.max(e["rp_ss"])), float(numpy.max(e["rp_sb"])),
@@ -259,13 +267,14 @@ class CuPyRAM:
             zmplt_e = self._per_env(self._zmplt, ne)
         nzplt_e = numpy.floor(zmplt_e / dz_e).astype(numpy.int64)
         self._zmplt_e = zmplt_e
-        nvr_e = numpy.floor(rmax_e / (dr_e * self._ndr)).astype(numpy.int64)
-        rmax_trunc = nvr_e * dr_e * self._ndr
+        ndr_e, ndz_e = self._ndr_e, self._ndz_e
+        nvr_e = numpy.floor(rmax_e / (dr_e * ndr_e)).astype(numpy.int64)
+        rmax_trunc = nvr_e * dr_e * ndr_e
         nsteps_e = numpy.round(rmax_trunc / dr_e).astype(numpy.int64) - 1
-        nvz_e = numpy.floor(nzplt_e / self._ndz).astype(numpy.int64)
+        nvz_e = numpy.floor(nzplt_e / ndz_e).astype(numpy.int64)
         if numpy.any(nz_e < 3):
             raise ValueError("depth grid too coarse (nz < 3)")
-        if numpy.any(nvz_e * self._ndz > nz_e + 1):
+        if numpy.any(nvz_e * ndz_e > nz_e + 1):
             raise ValueError("zmplt lies below the computational grid")
         # environment tables, extended bathymetry (cupyram.py:536-568)
         chunks, envs, off = [], numpy.zeros(ne, _abi.ENV_DTYPE), 0
@@ -317,12 +326,12 @@ class CuPyRAM:
         rays["k0"] = rays["omega"] / rays["c0"]
         rays["lyrw_lambda"] = self._lyrw * lam_e[env_of]
         rays["rs"] = rs_e[env_of]
-        rays["zs"] = self._zs
+        rays["zs"] = self._zs_e[env_of]
         rays["zr"] = self._zr
         rays["nsteps"] = nsteps_e[env_of]
         rays["iz0"] = iz0_e[env_of]
-        rays["ndr"] = self._ndr
-        rays["ndz"] = self._ndz
+        rays["ndr"] = ndr_e[env_of]
+        rays["ndz"] = ndz_e[env_of]
         rays["nvr"] = nvr_e[env_of]
         rays["nvz"] = nvz_e[env_of]
         rays["np"] = self._np
@@ -345,11 +354,11 @@ class CuPyRAM:
             self._rmax = float(rmax_trunc[0])
         nvr, nvz = int(nvr_e.max()), int(nvz_e.max())
         if self._per_ray_grid:
-            self.vr = [numpy.arange(1, n + 1) * d * self._ndr for n, d in zip(nvr_e, dr_e)]
-            self.vz = [numpy.arange(1, n + 1) * d * self._ndz for n, d in zip(nvz_e, dz_e)]
+            self.vr = [numpy.arange(1, n + 1) * d * m for n, d, m in zip(nvr_e, dr_e, ndr_e)]
+            self.vz = [numpy.arange(1, n + 1) * d * m for n, d, m in zip(nvz_e, dz_e, ndz_e)]
         else:
-            self.vr = numpy.arange(1, nvr + 1) * dr_e[0] * self._ndr
-            self.vz = numpy.arange(1, nvz + 1) * dz_e[0] * self._ndz
+            self.vr = numpy.arange(1, nvr + 1) * dr_e[0] * ndr_e[0]
+            self.vz = numpy.arange(1, nvz + 1) * dz_e[0] * ndz_e[0]
         self._nvr, self._nvz = nvr, nvz
         self.eta = 1 / (40 * numpy.pi * numpy.log10(numpy.exp(1)))
 

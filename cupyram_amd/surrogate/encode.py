@@ -1,0 +1,125 @@
+"""Query parameters of the surrogate and their encoding as network input.
+
+A query is the 101-number description of one TL field:
+    ssp     [32]  sound speed (m/s) at 32 evenly spaced depths over [0, ssp_depth]
+    bathy   [64]  water depth (m) at 64 evenly spaced ranges over [0, rmax]
+    bottom  [3]   three bottom parameters of the domain (see Domain.bottom_names)
+    freq          source frequency (Hz)
+    zs            source depth (m)
+and the answer is TL (dB) on the output grid of the domain: nr ranges
+r_j = (j+1) rmax/nr and nzp depths z_i = (i+1) zplot/nzp (the TL Grid of a
+march with ndr, ndz chosen so that its output points are exactly these).
+
+The encoding gives every (range bin, depth point) the 101 normalised query
+numbers plus 7 local ones (range, depth, local water depth, height above
+the seabed, offset from the source depth, a below-seabed flag, a constant),
+zero-padded to `features` columns; one network sample is one range bin.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import torch
+
+N_SSP, N_BATHY, N_GLOBAL, N_LOCAL = 32, 64, 101, 7
+
+
+@dataclasses.dataclass(frozen=True)
+class Domain:
+    name: str
+    rmax: float              # range of the last output bin (m)
+    zplot: float             # depth of the last output point (m)
+    ssp_depth: float         # depth of the last sound-speed knot (m)
+    bottom_mean: tuple       # centring of the 3 bottom parameters
+    bottom_scale: tuple
+    bottom_names: tuple
+    freq_lo: float
+    freq_hi: float
+    nr: int = 200
+    nzp: int = 64
+    c_ref: float = 1500.0
+    c_scale: float = 25.0
+    tl_lo: float = 20.0      # TL targets are clipped to [tl_lo, tl_hi] dB
+    tl_hi: float = 140.0
+    tl_mean: float = 80.0    # normalised target = (TL - tl_mean) / tl_std
+    tl_std: float = 20.0
+
+    def ranges(self):
+        return (np.arange(self.nr) + 1.0) * (self.rmax / self.nr)
+
+    def depths(self):
+        return (np.arange(self.nzp) + 1.0) * (self.zplot / self.nzp)
+
+
+@dataclasses.dataclass
+class Queries:
+    ssp: np.ndarray      # [n, 32]
+    bathy: np.ndarray    # [n, 64]
+    bottom: np.ndarray   # [n, 3]
+    freq: np.ndarray     # [n]
+    zs: np.ndarray       # [n]
+
+    def __len__(self):
+        return len(self.freq)
+
+    def take(self, idx):
+        return Queries(self.ssp[idx], self.bathy[idx], self.bottom[idx], self.freq[idx], self.zs[idx])
+
+    def vector(self):
+        """The 101 query numbers, [n, 101]."""
+        return np.concatenate([self.ssp, self.bathy, self.bottom, self.freq[:, None], self.zs[:, None]], axis=1)
+
+
+def global_features(dom: Domain, q: Queries, device="cuda"):
+    """The 101 normalised query numbers, float32 [n, 101]."""
+    dev = torch.device(device)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev)  # noqa: E731
+    return torch.cat([
+        (t(q.ssp) - dom.c_ref) / dom.c_scale,
+        t(q.bathy) / dom.zplot,
+        (t(q.bottom) - t(dom.bottom_mean)) / t(dom.bottom_scale),
+        ((torch.log(t(q.freq)) - np.log(dom.freq_lo)) / np.log(dom.freq_hi / dom.freq_lo))[:, None],
+        (t(q.zs) / dom.zplot)[:, None]], dim=1)
+
+
+def encode(dom: Domain, q: Queries, features: int = 128, device="cuda"):
+    """Network input, bf16 [n*nr, nzp, features]."""
+    n = len(q)
+    if features < N_GLOBAL + N_LOCAL:
+        raise ValueError("features too small")
+    dev = torch.device(device)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev)  # noqa: E731
+    g = global_features(dom, q, device)                               # [n, 101]
+    r = t(dom.ranges() / dom.rmax)                                    # [nr]
+    z = t(dom.depths() / dom.zplot)                                   # [nzp]
+    # water depth at each output range, linear between the 64 samples
+    pos = r * (N_BATHY - 1)
+    i0 = torch.clamp(pos.floor().long(), 0, N_BATHY - 2)
+    w = (pos - i0.float())[None, :]
+    bz = g[:, N_SSP:N_SSP + N_BATHY]
+    zb = bz[:, i0] * (1 - w) + bz[:, i0 + 1] * w                      # [n, nr] (in zplot units)
+    zsn = g[:, N_GLOBAL - 1]
+    out = torch.zeros(n, dom.nr, dom.nzp, features, device=dev, dtype=torch.float32)
+    out[..., :N_GLOBAL] = g[:, None, None, :]
+    out[..., N_GLOBAL + 0] = r[None, :, None]
+    out[..., N_GLOBAL + 1] = z[None, None, :]
+    out[..., N_GLOBAL + 2] = zb[:, :, None]
+    out[..., N_GLOBAL + 3] = z[None, None, :] - zb[:, :, None]
+    out[..., N_GLOBAL + 4] = z[None, None, :] - zsn[:, None, None]
+    out[..., N_GLOBAL + 5] = (z[None, None, :] > zb[:, :, None]).float()
+    out[..., N_GLOBAL + 6] = 1.0
+    return out.to(torch.bfloat16).reshape(n * dom.nr, dom.nzp, features)
+
+
+def normalise_tl(dom: Domain, tl):
+    """(target, mask) from TL [.., nr, nzp] (dB): clipped, centred, scaled;
+    non-finite points are masked out."""
+    tl = torch.as_tensor(tl, dtype=torch.float32)
+    mask = torch.isfinite(tl).float()
+    tl = torch.nan_to_num(tl, nan=dom.tl_hi, posinf=dom.tl_hi, neginf=dom.tl_lo)
+    return (tl.clamp(dom.tl_lo, dom.tl_hi) - dom.tl_mean) / dom.tl_std, mask
+
+
+def denormalise_tl(dom: Domain, y):
+    return y * dom.tl_std + dom.tl_mean

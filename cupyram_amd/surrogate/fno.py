@@ -1,0 +1,243 @@
+"""A 1-D Fourier-operator network over depth, trained and run on the
+library's BF16-MFMA kernels.
+
+One sample is one range bin of one query: a depth column of Z points with F
+input features per point.  The network is
+
+    h0     = x W0 + b0                                    (lift, F -> C)
+    h_l    = GELU(h_{l-1} Wp_l + bp_l + K_l(h_{l-1}))     l = 1..L
+    K_l(h) = irfft_Z( rfft_Z(h)[:Km] * W_l )              (W_l complex [Km][C][C])
+    q      = GELU(h_L P1 + c1)
+    y      = q P2 + c2                                    (C -> 1, TL in normalised dB)
+
+with rfft/irfft the real DFT pair over depth truncated to the first Km
+modes (the convention of numpy.fft.rfft / irfft).  Weights are kept in FP32
+(master copies and Adam moments) and used as BF16; every product accumulates
+in FP32.  Activations are stored as BF16 (inputs of the products) and FP32
+(pre-activations for the backward pass).
+
+The network has no counterpart in the reference; it is trained only on TL
+fields computed by this library's PE march.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+import torch
+
+from . import kernels as k
+
+
+@dataclasses.dataclass(frozen=True)
+class FNOConfig:
+    features: int = 128   # input features per depth point (padded)
+    width: int = 128      # channels C
+    modes: int = 16       # retained depth modes Km
+    layers: int = 4
+    depth_points: int = 64  # Z
+
+
+def dft_matrices(Z: int, Km: int):
+    """Analysis (rfft) and synthesis (irfft) matrices of the truncated real DFT.
+
+    analysis:  X_k = sum_z x_z (cos - i sin)(2 pi k z / Z)      -> (ac, as_) [Km][Z]
+    synthesis: x_z = (1/Z)(Re X_0 + 2 sum_{k>=1} Re(X_k e^{+2 pi i k z / Z}))
+                                                                -> (sc, ss) [Km][Z]
+    """
+    z = np.arange(Z)[None, :]
+    kk = np.arange(Km)[:, None]
+    th = 2.0 * np.pi * kk * z / Z
+    ac, as_ = np.cos(th), -np.sin(th)
+    w = np.where(kk == 0, 1.0 / Z, 2.0 / Z)
+    sc, ss = w * np.cos(th), -w * np.sin(th)
+    return ac, as_, sc, ss
+
+
+class FNO:
+    """Parameters, optimiser state and the forward/backward launch sequence."""
+
+    PARAM_ORDER = ("W0", "b0", *[f"{n}{l}" for l in range(1, 9) for n in ("Wr", "Wi", "Wp", "bp")],
+                   "P1", "c1", "P2", "c2")
+
+    def __init__(self, cfg: FNOConfig = FNOConfig(), seed: int = 0, device="cuda"):
+        if cfg.layers > 8:
+            raise ValueError("at most 8 layers")
+        self.cfg = cfg
+        self.device = torch.device(device)
+        F, Cw, Km, L = cfg.features, cfg.width, cfg.modes, cfg.layers
+        g = torch.Generator().manual_seed(seed)
+
+        def rnd(*shape, std):
+            return (torch.randn(*shape, generator=g, dtype=torch.float64) * std).float()
+
+        p = {"W0": rnd(F, Cw, std=1.0 / math.sqrt(F)), "b0": torch.zeros(Cw)}
+        for l in range(1, L + 1):
+            p[f"Wr{l}"] = rnd(Km, Cw, Cw, std=0.5 / Cw)
+            p[f"Wi{l}"] = rnd(Km, Cw, Cw, std=0.5 / Cw)
+            p[f"Wp{l}"] = rnd(Cw, Cw, std=1.0 / math.sqrt(Cw))
+            p[f"bp{l}"] = torch.zeros(Cw)
+        p["P1"] = rnd(Cw, Cw, std=1.0 / math.sqrt(Cw))
+        p["c1"] = torch.zeros(Cw)
+        p["P2"] = rnd(Cw, 1, std=1.0 / math.sqrt(Cw))
+        p["c2"] = torch.zeros(1)
+        self.names = [n for n in self.PARAM_ORDER if n in p]
+        self.params = {n: p[n].to(self.device).contiguous() for n in self.names}
+        self._alloc_state()
+        ac, as_, sc, ss = dft_matrices(cfg.depth_points, Km)
+        to = lambda a: torch.tensor(a, dtype=torch.float32, device=self.device).contiguous()  # noqa: E731
+        self.ac, self.as_, self.sc, self.ss = to(ac), to(as_), to(sc), to(ss)
+        self.step_count = 0
+        self._bufs = {}
+        self.refresh()
+
+    # -- parameters -------------------------------------------------------
+    def _alloc_state(self):
+        self.grads = {n: torch.zeros_like(t) for n, t in self.params.items()}
+        self.m1 = {n: torch.zeros_like(t) for n, t in self.params.items()}
+        self.m2 = {n: torch.zeros_like(t) for n, t in self.params.items()}
+        self.bf = {n: torch.empty(t.shape, dtype=torch.bfloat16, device=self.device)
+                   for n, t in self.params.items() if not n.startswith(("b", "c", "Wr", "Wi"))}
+        Km, Cw = self.cfg.modes, self.cfg.width
+        self.blk = {l: torch.empty(Km, 2 * Cw, 2 * Cw, dtype=torch.bfloat16, device=self.device)
+                    for l in range(1, self.cfg.layers + 1)}
+        self.gblk = {l: torch.zeros(Km, 2 * Cw, 2 * Cw, dtype=torch.float32, device=self.device)
+                     for l in range(1, self.cfg.layers + 1)}
+
+    def refresh(self):
+        """Refresh the BF16 copies from the FP32 master weights."""
+        for n, t in self.bf.items():
+            k.cast_bf16(self.params[n], t)
+        for l in range(1, self.cfg.layers + 1):
+            k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes, self.cfg.width)
+
+    def state_dict(self):
+        d = {n: t.detach().cpu().clone() for n, t in self.params.items()}
+        d["_config"] = torch.tensor([self.cfg.features, self.cfg.width, self.cfg.modes, self.cfg.layers,
+                                     self.cfg.depth_points])
+        return d
+
+    def load_state_dict(self, d):
+        for n in self.names:
+            self.params[n].copy_(d[n].to(self.device, torch.float32))
+        self.refresh()
+
+    def n_params(self):
+        return sum(t.numel() for t in self.params.values())
+
+    # -- buffers ----------------------------------------------------------
+    def _buf(self, name, shape, dtype):
+        t = self._bufs.get(name)
+        n = int(np.prod(shape))
+        if t is None or t.numel() < n or t.dtype != dtype:
+            t = torch.empty(n, dtype=dtype, device=self.device)
+            self._bufs[name] = t
+        return t[:n].view(*shape)
+
+    def release(self):
+        self._bufs.clear()
+
+    # -- forward ----------------------------------------------------------
+    def forward(self, x, train: bool = False):
+        """x: bf16 [S, Z, F] -> prediction f32 [S*Z] (normalised TL)."""
+        cfg = self.cfg
+        S, Z, F = x.shape
+        if Z != cfg.depth_points or F != cfg.features or x.dtype != torch.bfloat16:
+            raise ValueError(f"input must be bf16 [S, {cfg.depth_points}, {cfg.features}]")
+        x = x.contiguous()
+        Cw, Km, L = cfg.width, cfg.modes, cfg.layers
+        R = S * Z
+        bf, f32 = torch.bfloat16, torch.float32
+        self._S = S
+        self._x = x
+        h = [self._buf("h0", (R, Cw), bf)]
+        k.gemm(x, self.bf["W0"], h[0], R, Cw, F, bias=self.params["b0"], epi=k.EPI_BF16)
+        y = self._buf("y", (R, Cw), f32)
+        mh = self._buf("mh", (Km, S, 2 * Cw), bf)
+        for l in range(1, L + 1):
+            xh = self._buf(f"xh{l}" if train else "xh", (Km, S, 2 * Cw), bf)
+            k.spec_analysis(h[-1], xh, self.ac, self.as_, S, Z, Cw, Km)
+            k.gemm(xh, self.blk[l], mh, S, 2 * Cw, 2 * Cw, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+                   sc=S * 2 * Cw, epi=k.EPI_BF16)
+            k.spec_synth(mh, y, self.sc, self.ss, S, Z, Cw, Km)
+            hn = self._buf(f"h{l}" if train else f"h{l % 2}", (R, Cw), bf)
+            pre = self._buf(f"pre{l}", (R, Cw), f32) if train else None
+            k.gemm(h[-1], self.bf[f"Wp{l}"], hn, R, Cw, Cw, bias=self.params[f"bp{l}"], addend=y,
+                   pre=pre, epi=k.EPI_GELU_BF16)
+            h.append(hn)
+        q = self._buf("q", (R, Cw), bf)
+        preq = self._buf("preq", (R, Cw), f32) if train else None
+        k.gemm(h[-1], self.bf["P1"], q, R, Cw, Cw, bias=self.params["c1"], pre=preq, epi=k.EPI_GELU_BF16)
+        pred = self._buf("pred", (R,), f32)
+        k.gemm(q, self.bf["P2"], pred, R, 1, Cw, bias=self.params["c2"], epi=k.EPI_F32)
+        self._h = h
+        return pred
+
+    # -- loss and backward -----------------------------------------------
+    def loss_grad(self, pred, target, mask):
+        """Masked MSE; returns (device loss_sum[2], dpred bf16)."""
+        R = pred.numel()
+        cnt = float(mask.sum().item())
+        loss = self._buf("loss", (2,), torch.float32)
+        k.fill(loss)
+        dpred = self._buf("dpred", (R,), torch.bfloat16)
+        k.mse(pred, 1, target.reshape(-1), mask.reshape(-1), dpred, R, 2.0 / max(cnt, 1.0), loss)
+        return loss, dpred
+
+    def backward(self, dpred):
+        """Accumulate parameter gradients of the last train-mode forward."""
+        cfg = self.cfg
+        S, Z, F, Cw, Km, L = self._S, cfg.depth_points, cfg.features, cfg.width, cfg.modes, cfg.layers
+        R = S * Z
+        bf, f32 = torch.bfloat16, torch.float32
+        g = self.grads
+        for t in g.values():
+            k.fill(t)
+        for l in range(1, L + 1):
+            k.fill(self.gblk[l])
+        h = self._h
+        q = self._buf("q", (R, Cw), bf)
+        # y = q P2 + c2
+        k.gemm(q, dpred, g["P2"], Cw, 1, R, ta=True, epi=k.EPI_ATOMIC, splits=k.splits_for(Cw, 1, R))
+        k.colsum(dpred, g["c2"], R, 1)
+        dh = self._buf("dh", (R, Cw), f32)
+        k.gemm(dpred, self.bf["P2"], dh, R, Cw, 1, tb=True, epi=k.EPI_F32)
+        dpre = self._buf("dpre", (R, Cw), bf)
+        k.gelu_bwd(dh, self._buf("preq", (R, Cw), f32), dpre)
+        # q = GELU(h_L P1 + c1)
+        k.gemm(h[L], dpre, g["P1"], Cw, Cw, R, ta=True, epi=k.EPI_ATOMIC, splits=k.splits_for(Cw, Cw, R))
+        k.colsum(dpre, g["c1"], R, Cw)
+        k.gemm(dpre, self.bf["P1"], dh, R, Cw, Cw, tb=True, epi=k.EPI_F32)
+        dmh = self._buf("dmh", (Km, S, 2 * Cw), bf)
+        dxh = self._buf("dxh", (Km, S, 2 * Cw), bf)
+        for l in range(L, 0, -1):
+            k.gelu_bwd(dh, self._buf(f"pre{l}", (R, Cw), f32), dpre)
+            # pointwise branch
+            k.gemm(h[l - 1], dpre, g[f"Wp{l}"], Cw, Cw, R, ta=True, epi=k.EPI_ATOMIC,
+                   splits=k.splits_for(Cw, Cw, R))
+            k.colsum(dpre, g[f"bp{l}"], R, Cw)
+            k.gemm(dpre, self.bf[f"Wp{l}"], dh, R, Cw, Cw, tb=True, epi=k.EPI_F32)
+            # spectral branch: synthesis^T, mode mixing, analysis^T
+            k.spec_analysis(dpre, dmh, self.sc, self.ss, S, Z, Cw, Km)
+            xh = self._buf(f"xh{l}", (Km, S, 2 * Cw), bf)
+            k.gemm(xh, dmh, self.gblk[l], 2 * Cw, 2 * Cw, S, ta=True, batch=Km, sa=S * 2 * Cw, sb=S * 2 * Cw,
+                   sc=4 * Cw * Cw, epi=k.EPI_ATOMIC, splits=k.splits_for(2 * Cw, 2 * Cw, S, batch=Km))
+            k.gemm(dmh, self.blk[l], dxh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+                   sc=S * 2 * Cw, epi=k.EPI_BF16)
+            k.spec_synth(dxh, dh, self.ac, self.as_, S, Z, Cw, Km, accumulate=True)
+            k.spec_fold(self.gblk[l], g[f"Wr{l}"], g[f"Wi{l}"], Km, Cw)
+        # lift
+        k.gelu_bwd(dh, None, dpre)
+        k.gemm(self._x, dpre, g["W0"], F, Cw, R, ta=True, epi=k.EPI_ATOMIC, splits=k.splits_for(F, Cw, R))
+        k.colsum(dpre, g["b0"], R, Cw)
+
+    def step(self, lr, wd=0.0, b1=0.9, b2=0.999, eps=1e-8):
+        """One Adam step on every parameter from the accumulated gradients."""
+        self.step_count += 1
+        for n in self.names:
+            k.adam(self.params[n], self.grads[n], self.m1[n], self.m2[n], self.bf.get(n), lr, self.step_count,
+                   b1=b1, b2=b2, eps=eps, wd=0.0 if n.startswith(("b", "c")) else wd)
+        for l in range(1, self.cfg.layers + 1):
+            k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes,
+                          self.cfg.width)

@@ -1,0 +1,152 @@
+"""Checked launchers of the surrogate kernels (libcupyram_hip.so, cpr_fno_*).
+
+Tensors are torch device tensors (torch provides device memory and the
+stream; every operation here runs in the library's own kernels).  Each
+launcher verifies on the host that the operand extents cover every index the
+kernel will touch before it launches.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from .. import _abi
+
+EPI_F32, EPI_BF16, EPI_GELU_BF16, EPI_ATOMIC = 0, 1, 2, 3
+BK = 32
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _need(t, dtype, n, name):
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name}: contiguous device tensor expected")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: {dtype} expected, got {t.dtype}")
+    if t.numel() < n:
+        raise ValueError(f"{name}: {t.numel()} elements, kernel reaches {n}")
+
+
+def _extent(batch, stride, rows, ld, cols):
+    """Elements spanned by `batch` matrices of rows x cols (row stride ld)."""
+    return (batch - 1) * stride + (rows - 1) * ld + cols
+
+
+def gemm(a, b, c, M, N, K, *, ta=False, tb=False, epi=EPI_BF16, batch=1, lda=None, ldb=None, ldc=None,
+         sa=0, sb=0, sc=0, alpha=1.0, bias=None, addend=None, ldd=None, sd=0, pre=None, splits=1):
+    """c[b] = alpha op(a[b]) op(b[b]) (+bias) (+addend), see cpr_fno_gemm."""
+    lda = lda if lda is not None else (M if ta else K)
+    ldb = ldb if ldb is not None else (K if tb else N)
+    ldc = ldc if ldc is not None else N
+    ra, ca = (K, M) if ta else (M, K)
+    rb, cb = (N, K) if tb else (K, N)
+    _need(a, torch.bfloat16, _extent(batch, sa, ra, lda, ca), "gemm A")
+    _need(b, torch.bfloat16, _extent(batch, sb, rb, ldb, cb), "gemm B")
+    cdt = torch.float32 if epi in (EPI_F32, EPI_ATOMIC) else torch.bfloat16
+    _need(c, cdt, _extent(batch, sc, M, ldc, N), "gemm C")
+    if bias is not None:
+        _need(bias, torch.float32, N, "gemm bias")
+    if addend is not None:
+        ldd = ldd if ldd is not None else ldc
+        _need(addend, torch.float32, _extent(batch, sd, M, ldd, N), "gemm addend")
+    if pre is not None:
+        _need(pre, torch.float32, _extent(batch, sc, M, ldc, N), "gemm pre")
+    if splits > 1 and epi != EPI_ATOMIC:
+        raise ValueError("split-K needs the atomic epilogue")
+    _abi.check(_abi.lib().cpr_fno_gemm(int(ta), int(tb), epi, batch, M, N, K, float(alpha), _p(a), lda, sa,
+                                       _p(b), ldb, sb, _p(c), ldc, sc, _p(bias), _p(addend),
+                                       ldd if ldd is not None else 0, sd, _p(pre), int(splits), _stream()))
+
+
+def splits_for(M, N, K, batch=1, target_blocks=1024):
+    """Split-K factor that gives about target_blocks blocks."""
+    tiles = -(-M // 128) * -(-N // 128) * batch
+    s = max(1, min(target_blocks // max(tiles, 1), K // (4 * BK)))
+    return s
+
+
+def spec_analysis(x, out, bc, bs, S, Z, Cw, K):
+    if x.dtype not in (torch.float32, torch.bfloat16):
+        raise ValueError("spec_analysis: bf16 or f32 input")
+    _need(x, x.dtype, S * Z * Cw, "spec_analysis x")
+    _need(out, torch.bfloat16, K * S * 2 * Cw, "spec_analysis out")
+    _need(bc, torch.float32, K * Z, "spec_analysis bc")
+    _need(bs, torch.float32, K * Z, "spec_analysis bs")
+    _abi.check(_abi.lib().cpr_fno_spec_analysis(_p(x), int(x.dtype == torch.float32), _p(out), _p(bc), _p(bs),
+                                                S, Z, Cw, K, _stream()))
+
+
+def spec_synth(m, y, bc, bs, S, Z, Cw, K, accumulate=False):
+    _need(m, torch.bfloat16, K * S * 2 * Cw, "spec_synth m")
+    _need(y, torch.float32, S * Z * Cw, "spec_synth y")
+    _need(bc, torch.float32, K * Z, "spec_synth bc")
+    _need(bs, torch.float32, K * Z, "spec_synth bs")
+    _abi.check(_abi.lib().cpr_fno_spec_synth(_p(m), _p(y), _p(bc), _p(bs), S, Z, Cw, K, int(accumulate),
+                                             _stream()))
+
+
+def gelu_bwd(dy, pre, dx):
+    n = dy.numel()
+    _need(dy, torch.float32, n, "gelu_bwd dy")
+    if pre is not None:
+        _need(pre, torch.float32, n, "gelu_bwd pre")
+    _need(dx, torch.bfloat16, n, "gelu_bwd dx")
+    _abi.check(_abi.lib().cpr_fno_gelu_bwd(_p(dy), _p(pre), _p(dx), n, _stream()))
+
+
+def spec_expand(wr, wi, blk, K, Cw):
+    _need(wr, torch.float32, K * Cw * Cw, "spec_expand wr")
+    _need(wi, torch.float32, K * Cw * Cw, "spec_expand wi")
+    _need(blk, torch.bfloat16, K * 4 * Cw * Cw, "spec_expand blk")
+    _abi.check(_abi.lib().cpr_fno_spec_expand(_p(wr), _p(wi), _p(blk), K, Cw, _stream()))
+
+
+def spec_fold(g, gr, gi, K, Cw):
+    _need(g, torch.float32, K * 4 * Cw * Cw, "spec_fold g")
+    _need(gr, torch.float32, K * Cw * Cw, "spec_fold gr")
+    _need(gi, torch.float32, K * Cw * Cw, "spec_fold gi")
+    _abi.check(_abi.lib().cpr_fno_spec_fold(_p(g), _p(gr), _p(gi), K, Cw, _stream()))
+
+
+def colsum(x, out, M, N):
+    _need(x, torch.bfloat16, M * N, "colsum x")
+    _need(out, torch.float32, N, "colsum out")
+    _abi.check(_abi.lib().cpr_fno_colsum(_p(x), _p(out), M, N, _stream()))
+
+
+def mse(pred, ldp, target, mask, dpred, P, scale, loss_sum):
+    _need(pred, torch.float32, P * ldp, "mse pred")
+    _need(target, torch.float32, P, "mse target")
+    _need(mask, torch.float32, P, "mse mask")
+    _need(dpred, torch.bfloat16, P * ldp, "mse dpred")
+    _need(loss_sum, torch.float32, 2, "mse loss_sum")
+    _abi.check(_abi.lib().cpr_fno_mse(_p(pred), ldp, _p(target), _p(mask), _p(dpred), P, float(scale),
+                                      _p(loss_sum), _stream()))
+
+
+def adam(w, g, m1, m2, wb, lr, step, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    n = w.numel()
+    for t, name in ((w, "w"), (g, "g"), (m1, "m1"), (m2, "m2")):
+        _need(t, torch.float32, n, f"adam {name}")
+    if wb is not None:
+        _need(wb, torch.bfloat16, n, "adam wb")
+    _abi.check(_abi.lib().cpr_fno_adam(_p(w), _p(g), _p(m1), _p(m2), _p(wb), n, float(lr), float(b1), float(b2),
+                                       float(eps), int(step), float(wd), _stream()))
+
+
+def fill(x, v=0.0):
+    _need(x, torch.float32, x.numel(), "fill")
+    _abi.check(_abi.lib().cpr_fno_fill(_p(x), x.numel(), float(v), _stream()))
+
+
+def cast_bf16(x, y):
+    _need(x, torch.float32, x.numel(), "cast x")
+    _need(y, torch.bfloat16, x.numel(), "cast y")
+    _abi.check(_abi.lib().cpr_fno_cast_bf16(_p(x), _p(y), x.numel(), _stream()))

@@ -123,6 +123,54 @@ int cpr_team_shape(int precision, int nz, int *rows, int *lanes);
 /* Kernel variant and team shape chosen for ray b (for tests/autotuning). */
 int cpr_plan_describe(cpr_plan *p, int b, int *variant, int *team_rows, int *team_lanes);
 
+/* ------------------------------------------------------------------------
+ * Learned TL surrogate (a 1-D Fourier-operator network over depth).  These
+ * have no counterpart in the reference: they are the training and inference
+ * kernels of the surrogate that answers queries before the PE march does.
+ * Device pointers in, asynchronous on `stream` (a hipStream_t, NULL = the
+ * default stream); bf16 = IEEE bfloat16, row-major matrices.
+ * ------------------------------------------------------------------------ */
+enum { CPR_EPI_F32 = 0, CPR_EPI_BF16 = 1, CPR_EPI_GELU_BF16 = 2, CPR_EPI_ATOMIC = 3 };
+/* C[b] = alpha op(A[b]) op(B[b]) (+ bias[n]) (+ addend[b]) for b < batch,
+ * op(A)[m][k] = ta ? A[k*lda+m] : A[m*lda+k], op(B)[k][n] = tb ? B[n*ldb+k]
+ * : B[k*ldb+n] (bf16 operands, FP32 accumulation on MFMA).  Epilogues: f32
+ * store, bf16 store, GELU to bf16 (pre-activation stored to `pre` in C's
+ * layout when non-NULL), or FP32 atomic add (the only one allowed with
+ * splits > 1: K is split over `splits` blocks). */
+int cpr_fno_gemm(int ta, int tb, int epi, int batch, int M, int N, int K, float alpha, const void *A, int64_t lda,
+                 int64_t sA, const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC,
+                 const float *bias, const float *addend, int64_t ldd, int64_t sD, float *pre, int splits,
+                 void *stream);
+/* Truncated depth transform: x [S][Z][C] (bf16, or f32 if x_is_f32) ->
+ * out [K][S][2C] bf16 with out[k][s][c] = sum_z bc[k][z] x[s][z][c] and
+ * out[k][s][C+c] = sum_z bs[k][z] x[s][z][c].  Z <= 64, C <= 256, K <= 32. */
+int cpr_fno_spec_analysis(const void *x, int x_is_f32, void *out, const float *bc, const float *bs, int S, int Z,
+                          int C, int K, void *stream);
+/* Inverse: y[s][z][c] (+)= sum_k bc[k][z] m[k][s][c] + bs[k][z] m[k][s][C+c];
+ * m bf16, y f32. */
+int cpr_fno_spec_synth(const void *m, float *y, const float *bc, const float *bs, int S, int Z, int C, int K,
+                       int accumulate, void *stream);
+/* dx (bf16) = dy * GELU'(pre); pre = NULL gives dx = dy. */
+int cpr_fno_gelu_bwd(const float *dy, const float *pre, void *dx, int64_t n, void *stream);
+/* Complex mode weights Wr + i Wi [K][C][C] <-> real blocks [[Wr,Wi],[-Wi,Wr]]
+ * [K][2C][2C]: expand writes the bf16 block, fold maps a block gradient to
+ * the gradients of Wr and Wi. */
+int cpr_fno_spec_expand(const float *wr, const float *wi, void *blk, int K, int C, void *stream);
+int cpr_fno_spec_fold(const float *g, float *gr, float *gi, int K, int C, void *stream);
+/* out[n] += sum_m x[m][n], x bf16 [M][N]. */
+int cpr_fno_colsum(const void *x, float *out, int64_t M, int N, void *stream);
+/* Masked squared error on column 0 of pred [P][ldp]: dpred (bf16, same
+ * layout) = scale*mask*(pred-target) in column 0 and 0 elsewhere;
+ * loss_sum[0] += sum (mask*(pred-target))^2, loss_sum[1] += sum mask. */
+int cpr_fno_mse(const float *pred, int ldp, const float *target, const float *mask, void *dpred, int64_t P,
+                float scale, float *loss_sum, void *stream);
+/* Adam step `step` (1-based) on FP32 weights w with gradient g, moments m1,
+ * m2 and decoupled weight decay wd; refreshes the bf16 copy wb if non-NULL. */
+int cpr_fno_adam(float *w, const float *g, float *m1, float *m2, void *wb, int64_t n, float lr, float b1, float b2,
+                 float eps, int step, float wd, void *stream);
+int cpr_fno_fill(float *x, int64_t n, float v, void *stream);
+int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,14 +96,15 @@ def oracle_for(model, env_index=0, freq_index=0):
         f = float(model._freqs[freq_index])
     c = env_index * model._n_freq + freq_index
     ray = model._rays[c]
-    return O.OracleRay(freq=f, zs=model._zs, zr=model._zr, z_ss=e["z_ss"], rp_ss=e["rp_ss"], cw=e["cw"],
+    return O.OracleRay(freq=f, zs=float(model._zs_e[env_index]), zr=model._zr, z_ss=e["z_ss"], rp_ss=e["rp_ss"], cw=e["cw"],
                        z_sb=e["z_sb"], rp_sb=e["rp_sb"], cb=e["cb"], rhob=e["rhob"], attn=e["attn"],
                        rbzb=e["rbzb"], c0=float(ray["c0"]), lam=float(ray["lyrw_lambda"]) / model._lyrw,
                        dr=float(ray["dr"]), dz=float(ray["dz"]), zmax=float(ray["zmax"]),
                        rmax=float(np.asarray(model._rmax_input).ravel()[env_index]
                                   if np.ndim(model._rmax_input) else model._rmax_input),
                        zmplt=float(model._zmplt_e[env_index]), rs=float(ray["rs"]), np_pade=model._np, ns=model._ns,
-                       ndr=model._ndr, ndz=model._ndz, lyrw=float(model._lyrw))
+                       ndr=int(model._ndr_e[env_index]), ndz=int(model._ndz_e[env_index]),
+                       lyrw=float(model._lyrw))
 
 
 def tl_stats(a, b):
