@@ -32,6 +32,13 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mcell-steps/s (nz×range steps) c64 vs c128 at ≤0.5 dB TL err; HBM GB/s"
 
 
+# HBM bytes per cell-step of the complex64 team march on the 10,000-ray Munk
+# batch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy
+# (factors 2.0 and 1.0); keyed by (precision, team rows, team lanes).
+MEASURED_TRAFFIC = {(64, 24, 256): {"fetch": 170.4, "write": 4.1,
+                                     "source": "profiles/team_march_munk10k.md"}}
+
+
 def alg_bytes_per_cell_step(np_pade: int, precision: int) -> int:
     s = 8 if precision == 64 else 16     # complex element
     r = 4 if precision == 64 else 8      # real medium element
@@ -262,6 +269,8 @@ def main(argv=None):
     bytes_per_launch = alg_bytes_per_cell_step(np_pade, 64) * r64["cells"] * per_launch_steps
     avg_launch_s = r64["kernel_ms"] / max(r64["launches"], 1) / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    meas = MEASURED_TRAFFIC.get((64, r64["variant"][1], r64["variant"][2])) if r64["variant"][0] == 2 else None
+    traffic = (meas["fetch"] + meas["write"]) * r64["cells"] * per_launch_steps if meas else None
     line = {
         "metric": METRIC, "value": r64["mcs"], "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -272,9 +281,11 @@ def main(argv=None):
                    "rays": args.rays, "nz": int(r64["cells"] / args.rays), "np": np_pade,
                    "variant": r64["variant"], "steps_per_launch": per_launch_steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_cell_step": alg_bytes_per_cell_step(np_pade, 64),
-                     "kernel_ms_per_launch": avg_launch_s * 1e3},
+                     "kernel_ms_per_launch": avg_launch_s * 1e3,
+                     "traffic_source": meas["source"] if meas else None,
+                     "measured_hbm_gbs": traffic / avg_launch_s / 1e9 if traffic else None},
         "c128": out_c128,
     }
     if out_c128:

@@ -43,23 +43,35 @@ class RouterResult:
 
 class Router:
     def __init__(self, dom: Domain, members, threshold_db=3.0, spot_fraction=0.01, seed=0, precision="c64",
-                 batch=128, log=None):
+                 batch=128, log=None, pe=pe_fields, device="cuda"):
+        """members: trained networks (anything with forward(x, train=False)
+        returning normalised TL [n*nr*nzp]); pe(dom, queries, precision,
+        log=...) -> (TL [n, nr, nzp] dB, seconds, cell-steps) answers the
+        routed queries (the complex64 march by default)."""
         if not members:
             raise ValueError("the router needs at least one trained network")
+        if not 0.0 <= spot_fraction <= 1.0:
+            raise ValueError("spot_fraction must lie in [0, 1]")
         self.dom, self.members = dom, list(members)
         self.threshold, self.spot_fraction, self.seed = float(threshold_db), float(spot_fraction), seed
-        self.precision, self.batch, self.log = precision, batch, log
+        self.precision, self.batch, self.log, self.pe = precision, batch, log, pe
+        self.device = torch.device(device)
+
+    def _sync(self):
+        if self.device.type == "cuda":
+            torch.cuda.synchronize()
 
     def ensemble(self, q: Queries):
         """Ensemble mean TL [n, nr, nzp] and disagreement [n] (dB)."""
         dom, n = self.dom, len(q)
-        mean = torch.empty(n, dom.nr, dom.nzp, device="cuda")
-        spread = torch.empty(n, device="cuda")
+        dev = self.device
+        mean = torch.empty(n, dom.nr, dom.nzp, device=dev)
+        spread = torch.empty(n, device=dev)
         feats = self.members[0].cfg.features
         for a in range(0, n, self.batch):
             idx = np.arange(a, min(n, a + self.batch))
-            x = encode(dom, q.take(idx), feats)
-            s1 = torch.zeros(len(idx), dom.nr, dom.nzp, device="cuda")
+            x = encode(dom, q.take(idx), feats, device=dev)
+            s1 = torch.zeros(len(idx), dom.nr, dom.nzp, device=dev)
             s2 = torch.zeros_like(s1)
             for m in self.members:
                 y = denormalise_tl(dom, m.forward(x, train=False).view(len(idx), dom.nr, dom.nzp))
@@ -76,7 +88,7 @@ class Router:
         n = len(q)
         t0 = time.perf_counter()
         mean, spread = self.ensemble(q)
-        torch.cuda.synchronize()
+        self._sync()
         t_sur = time.perf_counter() - t0
         spread_h = spread.cpu().numpy()
         routed = spread_h > self.threshold
@@ -90,21 +102,21 @@ class Router:
         spot_err = np.zeros(0)
         batch_fallback = False
         if len(pe_idx):
-            pe_tl, _, _ = pe_fields(self.dom, q.take(pe_idx), self.precision, log=self.log)
+            pe_tl, _, _ = self.pe(self.dom, q.take(pe_idx), self.precision, log=self.log)
             pos = np.searchsorted(pe_idx, spot)
             if len(spot):
-                spot_err = field_rms(self.dom, mean[torch.as_tensor(spot, device="cuda")],
-                                     pe_tl[torch.as_tensor(pos, device="cuda")]).cpu().numpy()
-            tl[torch.as_tensor(pe_idx, device="cuda")] = pe_tl
+                spot_err = field_rms(self.dom, mean[torch.as_tensor(spot, device=self.device)],
+                                     pe_tl[torch.as_tensor(pos, device=self.device)]).cpu().numpy()
+            tl[torch.as_tensor(pe_idx, device=self.device)] = pe_tl
             routed[spot[spot_err > self.threshold]] = True
             batch_fallback = bool(len(spot_err) and np.sqrt(np.mean(spot_err ** 2)) > self.threshold)
         if batch_fallback:
             rest = np.nonzero(~np.isin(np.arange(n), pe_idx))[0]
             if len(rest):
-                rest_tl, _, _ = pe_fields(self.dom, q.take(rest), self.precision, log=self.log)
-                tl[torch.as_tensor(rest, device="cuda")] = rest_tl
+                rest_tl, _, _ = self.pe(self.dom, q.take(rest), self.precision, log=self.log)
+                tl[torch.as_tensor(rest, device=self.device)] = rest_tl
             routed[:] = True
-        torch.cuda.synchronize()
+        self._sync()
         t_pe = time.perf_counter() - t1
         return RouterResult(tl=tl, routed=routed, spread_db=spread_h, spot=spot, spot_err_db=spot_err,
                             batch_fallback=batch_fallback,

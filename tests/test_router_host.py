@@ -1,0 +1,93 @@
+"""CPU tests of the surrogate router: forced ensemble disagreement and forced
+surrogate bias above 3 dB must route queries to the PE march (here a stand-in
+PE whose answer is known), and accepted queries keep the ensemble mean."""
+import types
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from cupyram_amd.surrogate import dataset as D  # noqa: E402
+from cupyram_amd.surrogate.encode import global_features  # noqa: E402
+from cupyram_amd.surrogate.router import Router  # noqa: E402
+
+DOM = D.WEDGE
+FREQ = 99          # column of the normalised log-frequency in the query vector
+
+
+class Member:
+    """Predicts normalised TL = (log-frequency feature) + offset(query)."""
+
+    def __init__(self, offset=0.0, only_above=None):
+        self.cfg = types.SimpleNamespace(features=128)
+        self.offset, self.only_above = offset, only_above
+
+    def forward(self, x, train=False):
+        f = x[..., FREQ].float()                       # [n*nr, nzp]
+        off = torch.full_like(f, self.offset)
+        if self.only_above is not None:
+            off = torch.where(f > self.only_above, off, torch.zeros_like(off))
+        return (f + off).reshape(-1)
+
+
+def truth(q):
+    f = global_features(DOM, q, device="cpu")[:, FREQ].to(torch.bfloat16).float()
+    return (DOM.tl_mean + DOM.tl_std * f)[:, None, None].expand(len(q), DOM.nr, DOM.nzp).clone()
+
+
+def make_pe(bias_db=0.0, calls=None):
+    def pe(dom, q, precision, log=None):
+        if calls is not None:
+            calls.append(len(q))
+        return truth(q) - bias_db, 0.0, 0.0
+    return pe
+
+
+def test_agreeing_ensemble_is_accepted_and_spot_checked():
+    q = D.wedge_queries(300, 11)
+    calls = []
+    r = Router(DOM, [Member() for _ in range(4)], pe=make_pe(0.5, calls), device="cpu", batch=64)
+    res = r.run(q)
+    assert not res.routed.any() and not res.batch_fallback
+    assert len(res.spot) == 3 and calls == [3]                 # 1 % of the accepted queries
+    np.testing.assert_allclose(res.spot_err_db, 0.5, atol=0.02)
+    others = np.setdiff1d(np.arange(300), res.spot)
+    t = truth(q)
+    assert torch.allclose(res.tl[others], t[others], atol=0.1)
+    assert torch.allclose(res.tl[res.spot], t[res.spot] - 0.5, atol=1e-6)
+
+
+def test_disagreement_above_3db_routes_to_pe():
+    q = D.wedge_queries(300, 12)
+    f = global_features(DOM, q, device="cpu")[:, FREQ].numpy()
+    # one member 10 dB off on the upper half of the band: std of {0,0,0,10} = 4.3 dB
+    members = [Member(), Member(), Member(), Member(offset=0.5, only_above=0.5)]
+    r = Router(DOM, members, pe=make_pe(0.0), device="cpu", batch=64)
+    res = r.run(q)
+    hi = f > 0.5 + 0.01
+    lo = f < 0.5 - 0.01
+    assert res.routed[hi].all() and not res.routed[lo].any()
+    assert np.all(res.spread_db[hi] > 3.0) and np.all(res.spread_db[lo] < 1e-3)
+    t = truth(q)
+    assert torch.allclose(res.tl[torch.as_tensor(np.nonzero(hi)[0])], t[torch.as_tensor(np.nonzero(hi)[0])])
+    assert set(res.spot).isdisjoint(np.nonzero(res.spread_db > 3.0)[0])
+
+
+def test_biased_ensemble_fails_the_spot_check_and_the_batch_falls_back():
+    q = D.wedge_queries(400, 13)
+    calls = []
+    # every member agrees on an answer 5 dB too high: no disagreement, but the PE judges it wrong
+    r = Router(DOM, [Member(offset=0.25) for _ in range(4)], pe=make_pe(0.0, calls), device="cpu", batch=64)
+    res = r.run(q)
+    assert res.batch_fallback and res.routed.all()
+    assert np.all(res.spot_err_db > 3.0)
+    assert sum(calls) == 400
+    assert torch.allclose(res.tl, truth(q))
+
+
+def test_router_rejects_bad_configuration():
+    with pytest.raises(ValueError):
+        Router(DOM, [], device="cpu")
+    with pytest.raises(ValueError):
+        Router(DOM, [Member()], spot_fraction=1.5, device="cpu")
