@@ -217,6 +217,63 @@ def bench_wedge(args):
     return 0
 
 
+def fno_flops(cfg, S):
+    """Multiply-add flops (x2) of one forward pass over S samples: products on
+    MFMA, depth transforms on the vector ALUs."""
+    R = S * cfg.depth_points
+    C, K, F = cfg.width, cfg.modes, cfg.features
+    mfma = 2 * R * F * C + cfg.layers * (2 * K * S * (2 * C) ** 2 + 2 * R * C * C) + 2 * R * C * C + 2 * R * C
+    valu = cfg.layers * 2 * (2 * S * cfg.depth_points * C * 2 * K)
+    return mfma, valu
+
+
+def bench_fno(args):
+    """Forward (inference) and training-step throughput of the surrogate
+    network on synthetic bf16 inputs (64 queries x 200 range bins)."""
+    import torch
+    from cupyram_amd.surrogate import FNO, FNOConfig
+    cfg = FNOConfig()
+    nq = 64
+    S = nq * 200
+    g = torch.Generator().manual_seed(0)
+    x = (torch.randn(S, cfg.depth_points, cfg.features, generator=g) * 0.5).to(torch.bfloat16).cuda()
+    tgt = torch.randn(S * cfg.depth_points, generator=g).cuda()
+    mask = torch.ones_like(tgt)
+    fno = FNO(cfg, seed=0)
+    mf, vf = fno_flops(cfg, S)
+
+    def fwd():
+        fno.forward(x, train=False)
+
+    def train():
+        pred = fno.forward(x, train=True)
+        _, dpred = fno.loss_grad(pred, tgt, mask)
+        fno.backward(dpred)
+        fno.step(1e-4)
+
+    out = {}
+    for name, fn, n in (("inference", fwd, args.steps), ("train_step", train, max(1, args.steps // 3))):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / n
+        mult = 1 if name == "inference" else 3
+        out[name] = {"ms": dt * 1e3, "fields_per_s": nq / dt, "mfma_tflops": mult * mf / dt / 1e12,
+                     "valu_tflops": mult * vf / dt / 1e12}
+    line = {"metric": "surrogate fields/s (200 x 64 TL field per query)", "value": out["inference"]["fields_per_s"],
+            "unit": "fields/s", "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 (FP32 accumulation)",
+            "data": "synthetic inputs", "steps": args.steps,
+            "config": {"workload": "FNO 4 layers, width 128, 16 modes over 64 depths, 200 range bins, batch 64",
+                       "params": fno.n_params(), "forward_gflop_per_field": (mf + vf) / nq / 1e9},
+            **out}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,7 +287,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--cpu-rmax", type=float, default=5000.0)
-    ap.add_argument("--workload", default="munk", choices=["munk", "uneven", "wedge"])
+    ap.add_argument("--workload", default="munk", choices=["munk", "uneven", "wedge", "fno"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (torch) so that HBM counters can be scaled")
     args = ap.parse_args(argv)
@@ -247,6 +304,8 @@ def main(argv=None):
         del src, dst
     if args.workload == "wedge":
         return bench_wedge(args)
+    if args.workload == "fno":
+        return bench_fno(args)
     if args.workload == "uneven":
         if args.rays == 10000:
             args.rays = 50000

@@ -140,6 +140,20 @@ def summary(x):
             "p90_field_rms_db": float(np.percentile(x, 90)), "frac_fields_under_3db": float(np.mean(x < 3.0))}
 
 
+def by_band(dom, q, err, nbands=6):
+    """Per-field RMS error summarised over log-spaced frequency bands."""
+    e = err.detach().cpu().numpy()
+    edges = np.exp(np.linspace(np.log(dom.freq_lo), np.log(dom.freq_hi), nbands + 1))
+    out = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        m = (q.freq >= lo) & (q.freq <= hi)
+        if m.any():
+            out.append({"freq_lo": float(lo), "freq_hi": float(hi), "fields": int(m.sum()),
+                        "rms_db": float(np.sqrt(np.mean(e[m] ** 2))), "median_db": float(np.median(e[m])),
+                        "frac_under_3db": float(np.mean(e[m] < 3.0))})
+    return out
+
+
 def save_member(fno, dom, path):
     from safetensors.torch import save_file
     d = {}
@@ -181,7 +195,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=100)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--eval-only", action="store_true",
+                    help="evaluate the trained members found under --out on fresh validation fields")
     args = ap.parse_args(argv)
+    if args.eval_only:
+        return evaluate(args)
     dom = D.DOMAINS[args.domain]
     gen = D.deep_queries if dom.name == "deep_munk" else D.wedge_queries
     t0 = time.perf_counter()
@@ -212,7 +230,9 @@ def main(argv=None):
         del fno
     ens = torch.stack(preds).mean(0)
     spread = torch.sqrt(torch.stack(preds).var(0, unbiased=False).mean(dim=(1, 2)))
-    rep["ensemble_val"] = summary(field_rms(dom, ens, tl_va))
+    ens_err = field_rms(dom, ens, tl_va)
+    rep["ensemble_val"] = summary(ens_err)
+    rep["ensemble_val_by_band"] = by_band(dom, q_va, ens_err)
     rep["ensemble_spread_median_db"] = float(spread.median())
     rep["nearest_neighbour_val"] = summary(field_rms(dom, nearest_neighbour(dom, q_tr, tl_tr, q_va), tl_va))
     mean_field = tl_tr.mean(0, keepdim=True).expand_as(tl_va)
@@ -222,6 +242,26 @@ def main(argv=None):
     if args.report:
         with open(args.report, "w") as fh:
             json.dump(rep, fh, indent=1)
+
+
+def evaluate(args):
+    dom = D.DOMAINS[args.domain]
+    gen = D.deep_queries if dom.name == "deep_munk" else D.wedge_queries
+    members = load_ensemble(dom.name, args.out)
+    q_va = gen(args.n_val, args.seed + 1)
+    tl_va, t_va, c_va = pe_fields(dom, q_va)
+    preds = [predict(dom, m, q_va) for m in members]
+    ens = torch.stack(preds).mean(0)
+    err = field_rms(dom, ens, tl_va)
+    spread = torch.sqrt(torch.stack(preds).var(0, unbiased=False).mean(dim=(1, 2)))
+    rep = {"domain": dom.name, "n_val": args.n_val, "members": len(members), "ensemble_val": summary(err),
+           "ensemble_val_by_band": by_band(dom, q_va, err),
+           "spread_vs_error_corr": float(np.corrcoef(spread.cpu().numpy(), err.cpu().numpy())[0, 1])}
+    _log(json.dumps(rep, indent=1))
+    if args.report:
+        with open(args.report, "w") as fh:
+            json.dump(rep, fh, indent=1)
+    return 0
 
 
 if __name__ == "__main__":
