@@ -27,17 +27,40 @@
 
 namespace {
 
+// Reciprocal of a positive real: the hardware estimate for FP32 (1 ulp);
+// for FP64 the estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r), each of which squares the relative error.
 template <class R>
-__device__ inline cx<R> fast_rcp(cx<R> a);
+__device__ inline R rcp_real(R x);
 template <>
-__device__ inline cx<float> fast_rcp(cx<float> a) {
-  const float m = __builtin_amdgcn_rcpf(a.re * a.re + a.im * a.im);
-  return cxm<float>(a.re * m, -a.im * m);
+__device__ inline float rcp_real(float x) {
+  return __builtin_amdgcn_rcpf(x);
 }
 template <>
-__device__ inline cx<double> fast_rcp(cx<double> a) {
-  const double m = 1.0 / (a.re * a.re + a.im * a.im);
-  return cxm<double>(a.re * m, -a.im * m);
+__device__ inline double rcp_real(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/a as conj(a)/|a|^2
+template <class R>
+__device__ inline cx<R> fast_rcp(cx<R> a) {
+  const R m = rcp_real<R>(a.re * a.re + a.im * a.im);
+  return cxm<R>(a.re * m, -(a.im * m));
+}
+
+// -(a b), written so that the negation lands on FMA source operands
+template <class R>
+__device__ inline cx<R> cnmul(cx<R> a, cx<R> b) {
+  return cxm<R>(a.im * b.im - a.re * b.re, -(a.re * b.im) - a.im * b.re);
+}
+// a + b c
+template <class R>
+__device__ inline cx<R> cfma(cx<R> a, cx<R> b, cx<R> c) {
+  return cxm<R>(a.re + b.re * c.re - b.im * c.im, a.im + b.re * c.im + b.im * c.re);
 }
 
 // Copy a team-uniform value into scalar registers.
@@ -62,89 +85,107 @@ __device__ inline R shfl_down_r(R v, int s, int w) {
 }
 
 // Team exchange: values from lane t-s ("up") / t+s ("down"); zero outside
-// the team.  TW > 0: wave shuffles of width TW.  TW == 0: LDS with barriers.
+// the team.  TW > 0: wave shuffles of width TW.  TW == 0: LDS with one
+// barrier per exchange: the exchanges alternate between two buffers of 3*T
+// elements, so a buffer is written again only after every lane has passed
+// the barrier that follows its last reads.
+// B[i] if ok, else zero: the LDS load always reads an in-bounds element
+// (lane t's own) so that no pointer select between LDS and a stack slot is
+// formed.
+template <class R>
+__device__ inline cx<R> lds_or_zero(const cx<R> *B, int i, int t, bool ok) {
+  const cx<R> v = B[ok ? i : t];
+  return ok ? v : cxm<R>(0, 0);
+}
+
 template <class R, int TW>
 struct Team {
   int t, T;
-  cx<R> *lds;  // 3*T complex (LDS mode)
+  cx<R> *lds;  // 2 x 3*T complex (LDS mode)
+  int ph;
 
-  __device__ cx<R> up1(cx<R> v, int s) const {
+  __device__ cx<R> *next_buf() {
+    cx<R> *b = lds + ph * 3 * T;
+    ph ^= 1;
+    return b;
+  }
+  __device__ cx<R> up1(cx<R> v, int s) {
     if constexpr (TW > 0) {
       cx<R> r = cxm<R>(shfl_up_r(v.re, s, TW), shfl_up_r(v.im, s, TW));
       return t >= s ? r : cxm<R>(0, 0);
     } else {
+      cx<R> *B = next_buf();
+      B[t] = v;
       __syncthreads();
-      lds[t] = v;
-      __syncthreads();
-      return t >= s ? lds[t - s] : cxm<R>(0, 0);
-    }
-  }
-  __device__ cx<R> down1(cx<R> v, int s) const {
-    if constexpr (TW > 0) {
-      cx<R> r = cxm<R>(shfl_down_r(v.re, s, TW), shfl_down_r(v.im, s, TW));
-      return t + s < T ? r : cxm<R>(0, 0);
-    } else {
-      __syncthreads();
-      lds[t] = v;
-      __syncthreads();
-      return t + s < T ? lds[t + s] : cxm<R>(0, 0);
+      return lds_or_zero(B, t - s, t, t >= s);
     }
   }
   // halo: previous lane's `a` and next lane's `b`
-  __device__ void halo(cx<R> a, cx<R> b, cx<R> &from_prev, cx<R> &from_next) const {
+  __device__ void halo(cx<R> a, cx<R> b, cx<R> &from_prev, cx<R> &from_next) {
     if constexpr (TW > 0) {
       from_prev = up1(a, 1);
-      from_next = down1(b, 1);
+      cx<R> r = cxm<R>(shfl_down_r(b.re, 1, TW), shfl_down_r(b.im, 1, TW));
+      from_next = t + 1 < T ? r : cxm<R>(0, 0);
     } else {
+      cx<R> *B = next_buf();
+      B[t] = a;
+      B[T + t] = b;
       __syncthreads();
-      lds[t] = a;
-      lds[T + t] = b;
-      __syncthreads();
-      from_prev = t >= 1 ? lds[t - 1] : cxm<R>(0, 0);
-      from_next = t + 1 < T ? lds[T + t + 1] : cxm<R>(0, 0);
+      from_prev = lds_or_zero(B, t - 1, t, t >= 1);
+      from_next = lds_or_zero(B + T, t + 1, t, t + 1 < T);
     }
   }
-  __device__ void next3(cx<R> a, cx<R> b, cx<R> c, cx<R> &na, cx<R> &nb, cx<R> &nc) const {
+  __device__ void next3(cx<R> a, cx<R> b, cx<R> c, cx<R> &na, cx<R> &nb, cx<R> &nc) {
+    const bool ok = t + 1 < T;
+    const cx<R> z = cxm<R>(0, 0);
     if constexpr (TW > 0) {
-      na = down1(a, 1);
-      nb = down1(b, 1);
-      nc = down1(c, 1);
+      const cx<R> ra = cxm<R>(shfl_down_r(a.re, 1, TW), shfl_down_r(a.im, 1, TW));
+      const cx<R> rb = cxm<R>(shfl_down_r(b.re, 1, TW), shfl_down_r(b.im, 1, TW));
+      const cx<R> rc = cxm<R>(shfl_down_r(c.re, 1, TW), shfl_down_r(c.im, 1, TW));
+      na = ok ? ra : z;
+      nb = ok ? rb : z;
+      nc = ok ? rc : z;
     } else {
+      cx<R> *B = next_buf();
+      B[t] = a;
+      B[T + t] = b;
+      B[2 * T + t] = c;
       __syncthreads();
-      lds[t] = a;
-      lds[T + t] = b;
-      lds[2 * T + t] = c;
-      __syncthreads();
-      const bool ok = t + 1 < T;
-      na = ok ? lds[t + 1] : cxm<R>(0, 0);
-      nb = ok ? lds[T + t + 1] : cxm<R>(0, 0);
-      nc = ok ? lds[2 * T + t + 1] : cxm<R>(0, 0);
+      na = lds_or_zero(B, t + 1, t, ok);
+      nb = lds_or_zero(B + T, t + 1, t, ok);
+      nc = lds_or_zero(B + 2 * T, t + 1, t, ok);
     }
   }
   // one PCR exchange: (a, c, f) of lanes t-s and t+s
   __device__ void pcr(int s, cx<R> a, cx<R> c, cx<R> f, cx<R> &am, cx<R> &cm, cx<R> &fm, cx<R> &ap, cx<R> &cp,
-                      cx<R> &fp) const {
+                      cx<R> &fp) {
+    const bool lo = t >= s, hi = t + s < T;
+    const cx<R> z = cxm<R>(0, 0);
     if constexpr (TW > 0) {
-      am = up1(a, s);
-      cm = up1(c, s);
-      fm = up1(f, s);
-      ap = down1(a, s);
-      cp = down1(c, s);
-      fp = down1(f, s);
+      const cx<R> a1 = cxm<R>(shfl_up_r(a.re, s, TW), shfl_up_r(a.im, s, TW));
+      const cx<R> c1 = cxm<R>(shfl_up_r(c.re, s, TW), shfl_up_r(c.im, s, TW));
+      const cx<R> f1 = cxm<R>(shfl_up_r(f.re, s, TW), shfl_up_r(f.im, s, TW));
+      const cx<R> a2 = cxm<R>(shfl_down_r(a.re, s, TW), shfl_down_r(a.im, s, TW));
+      const cx<R> c2 = cxm<R>(shfl_down_r(c.re, s, TW), shfl_down_r(c.im, s, TW));
+      const cx<R> f2 = cxm<R>(shfl_down_r(f.re, s, TW), shfl_down_r(f.im, s, TW));
+      am = lo ? a1 : z;
+      cm = lo ? c1 : z;
+      fm = lo ? f1 : z;
+      ap = hi ? a2 : z;
+      cp = hi ? c2 : z;
+      fp = hi ? f2 : z;
     } else {
+      cx<R> *B = next_buf();
+      B[t] = a;
+      B[T + t] = c;
+      B[2 * T + t] = f;
       __syncthreads();
-      lds[t] = a;
-      lds[T + t] = c;
-      lds[2 * T + t] = f;
-      __syncthreads();
-      const bool lo = t >= s, hi = t + s < T;
-      const cx<R> z = cxm<R>(0, 0);
-      am = lo ? lds[t - s] : z;
-      cm = lo ? lds[T + t - s] : z;
-      fm = lo ? lds[2 * T + t - s] : z;
-      ap = hi ? lds[t + s] : z;
-      cp = hi ? lds[T + t + s] : z;
-      fp = hi ? lds[2 * T + t + s] : z;
+      am = lds_or_zero(B, t - s, t, lo);
+      cm = lds_or_zero(B + T, t - s, t, lo);
+      fm = lds_or_zero(B + 2 * T, t - s, t, lo);
+      ap = lds_or_zero(B, t + s, t, hi);
+      cp = lds_or_zero(B + T, t + s, t, hi);
+      fp = lds_or_zero(B + 2 * T, t + s, t, hi);
     }
   }
 };
@@ -299,7 +340,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   const cpr_ray P = make_uniform(a.rays[b]);
   const cpr_env E = make_uniform(a.envs[b]);
   RayState st = make_uniform(a.state[b]);
-  Team<R, TW> team{t, T, lds};
+  Team<R, TW> team{t, T, lds, 0};
   const int nz = P.nz;
   const int g0 = 1 + t * M;
   int mv = nz - g0 + 1;  // valid rows of this lane
@@ -320,8 +361,6 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   cx<R> *ug = static_cast<cx<R> *>(a.u);
 #pragma unroll
   for (int k = 0; k < M; ++k) L.u[k] = k < mv ? ug[static_cast<int64_t>(g0 + k) * a.ld + b] : cxm<R>(0, 0);
-  const R k0 = static_cast<R>(P.k0);
-  const R a1 = k0 * k0 / R(6), a2 = R(2) * k0 * k0 / R(3);
   // receiver rows: owner lane / slot of rows ir and ir+1
   const double ri = P.zr / P.dz;
   const int ir = static_cast<int>(floor(ri));
@@ -344,29 +383,48 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
     fresh = false;
     const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
     for (int j = 0; j < P.np; ++j) {
-      const cx<R> p1 = cxm<R>(static_cast<R>(pd[2 * j]), static_cast<R>(pd[2 * j + 1]));
-      const cx<R> p2 = cxm<R>(static_cast<R>(pd[2 * (P.np + j)]), static_cast<R>(pd[2 * (P.np + j) + 1]));
-      // 1. halo rows and downward elimination with the spike
+      // Pade factor (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1],
+      // divided by p2: (q [1 4 1] + D) x = (q [1 4 1] + rho D) u with
+      // q = k0^2 / (6 p2), rho = p1 / p2 (formed in FP64).
+      cx<R> q, q4, rho;
+      {
+        const double p1r = pd[2 * j], p1i = pd[2 * j + 1];
+        const double p2r = pd[2 * (P.np + j)], p2i = pd[2 * (P.np + j) + 1];
+        const double n2 = 1.0 / (p2r * p2r + p2i * p2i);
+        const double ir = p2r * n2, ii = -p2i * n2;
+        const double a1d = P.k0 * P.k0 / 6.0;
+        q = cxm<R>(static_cast<R>(a1d * ir), static_cast<R>(a1d * ii));
+        q4 = cxm<R>(static_cast<R>(4.0 * a1d * ir), static_cast<R>(4.0 * a1d * ii));
+        rho = cxm<R>(static_cast<R>(p1r * ir - p1i * ii), static_cast<R>(p1r * ii + p1i * ir));
+      }
+      // 1. halo rows and downward elimination with the spike.  Rows past nz
+      // (k >= mv) get a zero reciprocal pivot, so c', w and d' vanish there
+      // and stay zero through the upward sweep and the back substitution.
       cx<R> um, u_next;
       team.halo(L.u[M - 1], L.u[0], um, u_next);
-      cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(-1, 0), dprev = cxm<R>(0, 0);
+      cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(0, 0), dprev = cxm<R>(0, 0);
+      CoefRow<R> crn = C.get(0);
 #pragma unroll
       for (int k = 0; k < M; ++k) {
+        const CoefRow<R> cr = crn;
+        if (k + 1 < M) crn = C.get(k + 1);
         const cx<R> uc = L.u[k];
         const cx<R> up = (k + 1 < M) ? L.u[k + 1] : u_next;
-        const CoefRow<R> cr = C.get(k);
-        const cx<R> r1 = cxm<R>(a1, 0) + p2 * cr.d1;
-        const cx<R> r2 = cxm<R>(a2, 0) + p2 * cr.d2;
-        const cx<R> r3 = cxm<R>(a1, 0) + p2 * cr.d3;
-        const cx<R> du = cr.d1 * um + cr.d2 * uc + cr.d3 * up;
-        const cx<R> rhs = a1 * (um + up) + a2 * uc + p1 * du;
-        const cx<R> inv = fast_rcp(cfms(r2, r1, cprev));
-        cx<R> cc = r3 * inv;
-        cx<R> wk = cxm<R>(0, 0) - (r1 * wprev) * inv;
-        cx<R> dk = cfms(rhs, r1, dprev) * inv;
-        if (k >= mv) {
-          cc = wk = dk = cxm<R>(0, 0);
-        }
+        const cx<R> r1 = q + cr.d1;
+        const cx<R> r2 = q4 + cr.d2;
+        const cx<R> r3 = q + cr.d3;
+        const cx<R> s = um + up;
+        const cx<R> ts = cxm<R>(s.re + R(4) * uc.re, s.im + R(4) * uc.im);
+        const cx<R> du = cfma(cfma(cr.d1 * um, cr.d2, uc), cr.d3, up);
+        const cx<R> rhs = cfma(q * ts, rho, du);
+        const cx<R> den = k == 0 ? r2 : cfms(r2, r1, cprev);
+        R m = rcp_real<R>(den.re * den.re + den.im * den.im);
+        m = k < mv ? m : R(0);
+        const cx<R> inv = cxm<R>(den.re * m, -(den.im * m));
+        const cx<R> cc = r3 * inv;
+        // first row: the previous unknown is y_{t-1} itself (w = -1, c' = d' = 0)
+        const cx<R> wk = k == 0 ? r1 * inv : cnmul(r1 * wprev, inv);
+        const cx<R> dk = k == 0 ? rhs * inv : cfms(rhs, r1, dprev) * inv;
         X.set(k, cc, wk);
         L.u[k] = dk;
         cprev = cc;
@@ -379,15 +437,15 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
       for (int k = M - 3; k >= 0; --k) {
         const cx<R> ck = X.c(k);
         L.u[k] = cfms(L.u[k], ck, L.u[k + 1]);
-        X.set(k, cxm<R>(0, 0) - ck * X.c(k + 1), cfms(X.w(k), ck, X.w(k + 1)));
+        X.set(k, cnmul(ck, X.c(k + 1)), cfms(X.w(k), ck, X.w(k + 1)));
       }
       // 3. reduced equation a y_{t-1} + b y_t + c y_{t+1} = f, then PCR
       cx<R> nD, nW, nC;
       team.next3(L.u[0], X.w(0), X.c(0), nD, nW, nC);
       const cx<R> ce = X.c(M - 1);
       cx<R> ea = t > 0 ? X.w(M - 1) : cxm<R>(0, 0);
-      cx<R> eb = cxm<R>(1, 0) - ce * nW;
-      cx<R> ec = cxm<R>(0, 0) - ce * nC;
+      const cx<R> eb = cfms(cxm<R>(1, 0), ce, nW);
+      cx<R> ec = cnmul(ce, nC);
       cx<R> ef = cfms(L.u[M - 1], ce, nD);
       {
         const cx<R> ib = fast_rcp(eb);
@@ -398,24 +456,21 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
       for (int s = 1; s < T; s <<= 1) {
         cx<R> am, cm, fm, ap, cp, fp;
         team.pcr(s, ea, ec, ef, am, cm, fm, ap, cp, fp);
-        const cx<R> bn = cxm<R>(1, 0) - ea * cm - ec * ap;
-        const cx<R> an = cxm<R>(0, 0) - ea * am;
-        const cx<R> cn = cxm<R>(0, 0) - ec * cp;
-        const cx<R> fn = ef - ea * fm - ec * fp;
+        const cx<R> bn = cfms(cfms(cxm<R>(1, 0), ea, cm), ec, ap);
+        const cx<R> an = cnmul(ea, am);
+        const cx<R> cn = cnmul(ec, cp);
+        const cx<R> fn = cfms(cfms(ef, ea, fm), ec, fp);
         const cx<R> ib = fast_rcp(bn);
         ea = an * ib;
         ec = cn * ib;
         ef = fn * ib;
       }
-      // 4. back substitution
+      // 4. back substitution (rows past nz come out zero)
       const cx<R> yc = ef;
       const cx<R> yl = team.up1(yc, 1);
 #pragma unroll
-      for (int k = 0; k < M - 1; ++k) {
-        const cx<R> x = L.u[k] - X.w(k) * yl - X.c(k) * yc;
-        L.u[k] = k < mv ? x : cxm<R>(0, 0);
-      }
-      L.u[M - 1] = (M - 1) < mv ? yc : cxm<R>(0, 0);
+      for (int k = 0; k < M - 1; ++k) L.u[k] = cfms(cfms(L.u[k], X.w(k), yl), X.c(k), yc);
+      L.u[M - 1] = yc;
     }
     // output (outpt.py:11-58): complex pressure only; TL is formed by a
     // post-pass over the whole output array
@@ -438,20 +493,20 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
           ub = cxm<R>(__shfl(vb.re, tbs, TW), __shfl(vb.im, tbs, TW));
           f3b = __shfl(fb, tbs, TW);
         } else {
-          __syncthreads();
+          cx<R> *B = team.next_buf();
           if (t == ta) {
-            lds[0] = va;
-            lds[2] = cxm<R>(fa, 0);
+            B[0] = va;
+            B[2] = cxm<R>(fa, 0);
           }
           if (t == tb) {
-            lds[1] = vb;
-            lds[3] = cxm<R>(fb, 0);
+            B[1] = vb;
+            B[3] = cxm<R>(fb, 0);
           }
           __syncthreads();
-          ua = lds[0];
-          ub = lds[1];
-          f3a = lds[2].re;
-          f3b = lds[3].re;
+          ua = B[0];
+          ub = B[1];
+          f3a = B[2].re;
+          f3b = B[3].re;
         }
         if (ir < 1) {
           ua = cxm<R>(0, 0);
@@ -521,7 +576,7 @@ int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
     hipLaunchKernelGGL((k_march_team_w<R, M, DS, DX>), dim3(grid), dim3(256), 0, st, a, s0, s1);
   } else {
     if (lanes % 64 || lanes > 512) return -1;
-    const size_t lds = 3 * static_cast<size_t>(lanes) * sizeof(cx<R>);
+    const size_t lds = 6 * static_cast<size_t>(lanes) * sizeof(cx<R>);
     hipLaunchKernelGGL((k_march_team_l<R, M, DS, DX>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
