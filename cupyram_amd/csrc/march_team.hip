@@ -86,29 +86,48 @@ __device__ inline R shfl_down_r(R v, int s, int w) {
 
 // Team exchange: values from lane t-s ("up") / t+s ("down"); zero outside
 // the team.  TW > 0: wave shuffles of width TW.  TW == 0: LDS with one
-// barrier per exchange: the exchanges alternate between two buffers of 3*T
-// elements, so a buffer is written again only after every lane has passed
-// the barrier that follows its last reads.
-// B[i] if ok, else zero: the LDS load always reads an in-bounds element
-// (lane t's own) so that no pointer select between LDS and a stack slot is
-// formed.
+// barrier per exchange.  LDS layout: two buffers, used alternately, so a
+// buffer is written again only after every lane has passed the barrier that
+// follows its last reads of it; each buffer holds three arrays of
+// kPad + T + kPad elements whose kPad-element margins are zeroed once
+// (team_lds_init) and never written, so reads at distance s <= kPad need no
+// bounds test.  Wider distances (PCR levels s >= 64) are wave-uniform tests.
+constexpr int kPad = 32;
+
 template <class R>
 __device__ inline cx<R> lds_or_zero(const cx<R> *B, int i, int t, bool ok) {
   const cx<R> v = B[ok ? i : t];
   return ok ? v : cxm<R>(0, 0);
 }
 
+// LDS bytes of an LDS-mode team of T lanes
+template <class R>
+constexpr size_t team_lds_bytes(int T) {
+  return 2 * 3 * static_cast<size_t>(T + 2 * kPad) * sizeof(cx<R>);
+}
+
 template <class R, int TW>
 struct Team {
   int t, T;
-  cx<R> *lds;  // 2 x 3*T complex (LDS mode)
+  cx<R> *lds;  // LDS mode: 2 buffers x 3 arrays x (kPad + T + kPad)
   int ph;
 
+  // zero both buffers (margins included); one barrier
+  __device__ void init() {
+    if constexpr (TW == 0) {
+      const int n = 2 * 3 * (T + 2 * kPad);
+      for (int i = t; i < n; i += T) lds[i] = cxm<R>(0, 0);
+      __syncthreads();
+    }
+  }
+  // data start of array 0 of the next buffer; array j starts at + j * stride()
   __device__ cx<R> *next_buf() {
-    cx<R> *b = lds + ph * 3 * T;
+    cx<R> *b = lds + ph * 3 * (T + 2 * kPad) + kPad;
     ph ^= 1;
     return b;
   }
+  __device__ int stride() const { return T + 2 * kPad; }
+
   __device__ cx<R> up1(cx<R> v, int s) {
     if constexpr (TW > 0) {
       cx<R> r = cxm<R>(shfl_up_r(v.re, s, TW), shfl_up_r(v.im, s, TW));
@@ -117,7 +136,7 @@ struct Team {
       cx<R> *B = next_buf();
       B[t] = v;
       __syncthreads();
-      return lds_or_zero(B, t - s, t, t >= s);
+      return B[t - s];  // s = 1 here: the margin supplies the zero
     }
   }
   // halo: previous lane's `a` and next lane's `b`
@@ -128,17 +147,18 @@ struct Team {
       from_next = t + 1 < T ? r : cxm<R>(0, 0);
     } else {
       cx<R> *B = next_buf();
+      const int S = stride();
       B[t] = a;
-      B[T + t] = b;
+      B[S + t] = b;
       __syncthreads();
-      from_prev = lds_or_zero(B, t - 1, t, t >= 1);
-      from_next = lds_or_zero(B + T, t + 1, t, t + 1 < T);
+      from_prev = B[t - 1];
+      from_next = B[S + t + 1];
     }
   }
   __device__ void next3(cx<R> a, cx<R> b, cx<R> c, cx<R> &na, cx<R> &nb, cx<R> &nc) {
-    const bool ok = t + 1 < T;
-    const cx<R> z = cxm<R>(0, 0);
     if constexpr (TW > 0) {
+      const bool ok = t + 1 < T;
+      const cx<R> z = cxm<R>(0, 0);
       const cx<R> ra = cxm<R>(shfl_down_r(a.re, 1, TW), shfl_down_r(a.im, 1, TW));
       const cx<R> rb = cxm<R>(shfl_down_r(b.re, 1, TW), shfl_down_r(b.im, 1, TW));
       const cx<R> rc = cxm<R>(shfl_down_r(c.re, 1, TW), shfl_down_r(c.im, 1, TW));
@@ -147,13 +167,14 @@ struct Team {
       nc = ok ? rc : z;
     } else {
       cx<R> *B = next_buf();
+      const int S = stride();
       B[t] = a;
-      B[T + t] = b;
-      B[2 * T + t] = c;
+      B[S + t] = b;
+      B[2 * S + t] = c;
       __syncthreads();
-      na = lds_or_zero(B, t + 1, t, ok);
-      nb = lds_or_zero(B + T, t + 1, t, ok);
-      nc = lds_or_zero(B + 2 * T, t + 1, t, ok);
+      na = B[t + 1];
+      nb = B[S + t + 1];
+      nc = B[2 * S + t + 1];
     }
   }
   // one PCR exchange: (a, c, f) of lanes t-s and t+s
@@ -176,16 +197,26 @@ struct Team {
       fp = hi ? f2 : z;
     } else {
       cx<R> *B = next_buf();
+      const int S = stride();
       B[t] = a;
-      B[T + t] = c;
-      B[2 * T + t] = f;
+      B[S + t] = c;
+      B[2 * S + t] = f;
       __syncthreads();
-      am = lds_or_zero(B, t - s, t, lo);
-      cm = lds_or_zero(B + T, t - s, t, lo);
-      fm = lds_or_zero(B + 2 * T, t - s, t, lo);
-      ap = lds_or_zero(B, t + s, t, hi);
-      cp = lds_or_zero(B + T, t + s, t, hi);
-      fp = lds_or_zero(B + 2 * T, t + s, t, hi);
+      if (s <= kPad) {
+        am = B[t - s];
+        cm = B[S + t - s];
+        fm = B[2 * S + t - s];
+        ap = B[t + s];
+        cp = B[S + t + s];
+        fp = B[2 * S + t + s];
+      } else {
+        am = lds_or_zero(B, t - s, t, lo);
+        cm = lds_or_zero(B + S, t - s, t, lo);
+        fm = lds_or_zero(B + 2 * S, t - s, t, lo);
+        ap = lds_or_zero(B, t + s, t, hi);
+        cp = lds_or_zero(B + S, t + s, t, hi);
+        fp = lds_or_zero(B + 2 * S, t + s, t, hi);
+      }
     }
   }
 };
@@ -211,13 +242,14 @@ struct Elim {
 };
 template <class R, int M>
 struct Elim<R, M, true> {
-  cx<R> *base;  // row k at base[2 * k * T] (c) and base[2 * k * T + 1] (w)
-  int T;
-  __device__ cx<R> c(int k) const { return base[2 * static_cast<int64_t>(k) * T]; }
-  __device__ cx<R> w(int k) const { return base[2 * static_cast<int64_t>(k) * T + 1]; }
+  cx<R> *ray;  // team-uniform: row k of lane t at ray[2 * (k * T + t)] (c), + 1 (w)
+  int T, t;
+  __device__ cx<R> c(int k) const { return (ray + 2 * k * T)[2 * t]; }
+  __device__ cx<R> w(int k) const { return (ray + 2 * k * T)[2 * t + 1]; }
   __device__ void set(int k, cx<R> cv, cx<R> wv) {
-    base[2 * static_cast<int64_t>(k) * T] = cv;
-    base[2 * static_cast<int64_t>(k) * T + 1] = wv;
+    cx<R> *row = ray + 2 * k * T;
+    row[2 * t] = cv;
+    row[2 * t + 1] = wv;
   }
 };
 
@@ -238,9 +270,10 @@ struct Coefs {
 };
 template <class R, int M>
 struct Coefs<R, M, true> {
-  CoefRow<R> *base;  // row k at base[k * T]
-  int T;
-  __device__ CoefRow<R> get(int k) const { return base[static_cast<int64_t>(k) * T]; }
+  CoefRow<R> *ray;  // team-uniform: row k of lane t at ray[k * T + t]
+  int T, t;
+  __device__ CoefRow<R> get(int k) const { return (ray + k * T)[t]; }
+  __device__ CoefRow<R> &at(int k) const { return (ray + k * T)[t]; }
 };
 
 // Water and bottom media of the lane's rows g0-1 .. g0+M (FP64), kept in a
@@ -315,7 +348,7 @@ __device__ void rebuild_rows(Coefs<R, M, DS> &C, const MedCache<M> &mc, int g0, 
   khi = khi > M - 1 ? M - 1 : khi;
   if constexpr (DS) {
 #pragma unroll 1
-    for (int k = klo; k <= khi; ++k) C.base[static_cast<int64_t>(k) * C.T] = coef_from_cache<R, M>(mc, k, g0, iz, cfact);
+    for (int k = klo; k <= khi; ++k) C.at(k) = coef_from_cache<R, M>(mc, k, g0, iz, cfact);
   } else {
     CoefRow<R> tmp[M];
 #pragma unroll 1
@@ -341,6 +374,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   const cpr_env E = make_uniform(a.envs[b]);
   RayState st = make_uniform(a.state[b]);
   Team<R, TW> team{t, T, lds, 0};
+  team.init();
   const int nz = P.nz;
   const int g0 = 1 + t * M;
   int mv = nz - g0 + 1;  // valid rows of this lane
@@ -349,14 +383,16 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   Coefs<R, M, DS> C;
   Elim<R, M, DX> X;
   if constexpr (DX) {
-    X.base = static_cast<cx<R> *>(a.xbuf) + 2 * (static_cast<int64_t>(b) * M * T + t);
+    X.ray = static_cast<cx<R> *>(a.xbuf) + 2 * static_cast<int64_t>(b) * M * T;
     X.T = T;
+    X.t = t;
   }
   MedCache<M> mc;
   const double cfact = 0.5 / (P.dz * P.dz);
   if constexpr (DS) {
-    C.base = static_cast<CoefRow<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T + t;
+    C.ray = static_cast<CoefRow<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T;
     C.T = T;
+    C.t = t;
   }
   cx<R> *ug = static_cast<cx<R> *>(a.u);
 #pragma unroll
@@ -576,7 +612,7 @@ int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
     hipLaunchKernelGGL((k_march_team_w<R, M, DS, DX>), dim3(grid), dim3(256), 0, st, a, s0, s1);
   } else {
     if (lanes % 64 || lanes > 512) return -1;
-    const size_t lds = 6 * static_cast<size_t>(lanes) * sizeof(cx<R>);
+    const size_t lds = team_lds_bytes<R>(lanes);
     hipLaunchKernelGGL((k_march_team_l<R, M, DS, DX>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
