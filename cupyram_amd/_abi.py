@@ -43,7 +43,7 @@ class CprConfig(C.Structure):
     _fields_ = [
         ("nrays", C.c_int32), ("precision", C.c_int32), ("variant", C.c_int32),
         ("compute_grids", C.c_int32), ("nvr_max", C.c_int32), ("nvz_max", C.c_int32),
-        ("team_rows", C.c_int32), ("team_lanes", C.c_int32),
+        ("team_rows", C.c_int32), ("team_lanes", C.c_int32), ("compute_cells", C.c_int32),
     ]
 
 
@@ -80,6 +80,7 @@ _SYMBOLS = {
     "cpr_march_time": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
     "cpr_read_outputs": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "cpr_read_field": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "cpr_read_cells": ([C.c_void_p, C.c_void_p], C.c_int),
     "cpr_plan_device_bytes": ([C.c_void_p], C.c_int64),
     "cpr_team_shape": ([C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "cpr_plan_describe": ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
@@ -187,6 +188,12 @@ class Plan:
         cpg = np.zeros((n, nvz, nvr), np.complex128) if grids else None
         check(lib().cpr_read_outputs(self._h, ptr(tll), ptr(cpl), ptr(tlg), ptr(cpg)))
         return tll, cpl, tlg, cpg
+
+    def read_cells(self):
+        """Cell-mean TL [nrays, nvz_max, nvr_max] (plan made with compute_cells)."""
+        tlc = np.zeros((self.cfg.nrays, self.cfg.nvz_max, self.cfg.nvr_max))
+        check(lib().cpr_read_cells(self._h, ptr(tlc)))
+        return tlc
 
     def read_field(self, nz_max):
         u = np.zeros((self.cfg.nrays, nz_max + 2), np.complex128)

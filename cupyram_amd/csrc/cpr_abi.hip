@@ -57,6 +57,8 @@ struct cpr_plan {
   RayState *state_d = nullptr;
   void *col[7] = {};  // f1 f2 f3 ksq u cp dp
   double *tll_d = nullptr, *cpl_d = nullptr, *tlg_d = nullptr, *cpg_d = nullptr;
+  float *cells_d = nullptr;
+  double *tlcell_d = nullptr;
   int32_t *team_rays_d = nullptr;
   char *dbuf_d = nullptr;
   char *xbuf_d = nullptr;
@@ -91,6 +93,8 @@ struct cpr_plan {
     a.cpl = cpl_d;
     a.tlg = tlg_d;
     a.cpg = cpg_d;
+    a.cells = cells_d;
+    a.tlcell = tlcell_d;
     a.team_rays = team_rays_d;
     a.team_nrays = nrays;
     a.dbuf = dbuf_d;
@@ -207,7 +211,8 @@ int cpr_plan_destroy(cpr_plan *p) {
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
                   static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d),
                   static_cast<void *>(p->team_rays_d), static_cast<void *>(p->dbuf_d),
-                  static_cast<void *>(p->xbuf_d)})
+                  static_cast<void *>(p->xbuf_d), static_cast<void *>(p->cells_d),
+                  static_cast<void *>(p->tlcell_d)})
     if (q) (void)hipFree(q);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -229,7 +234,8 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     if (check_ray(rays[b], envs[b], env_len, b)) return -1;
     if (rays[b].np != np_) return fail("all rays of a plan must use the same np");
     if (rays[b].nvr > cfg->nvr_max) return fail("ray nvr exceeds nvr_max");
-    if (cfg->compute_grids && rays[b].nvz > cfg->nvz_max) return fail("ray nvz exceeds nvz_max");
+    if ((cfg->compute_grids || cfg->compute_cells) && rays[b].nvz > cfg->nvz_max)
+      return fail("ray nvz exceeds nvz_max");
     nz_max = std::max(nz_max, rays[b].nz);
     max_steps = std::max(max_steps, rays[b].nsteps);
   }
@@ -295,6 +301,10 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     const size_t ngrid = nline * static_cast<size_t>(std::max(cfg->nvz_max, 1));
     if (dalloc(p, &p->tlg_d, ngrid) || dalloc(p, &p->cpg_d, 2 * ngrid)) return bail(-1);
   }
+  if (cfg->compute_cells) {
+    const size_t ngrid = nline * static_cast<size_t>(std::max(cfg->nvz_max, 1));
+    if (dalloc(p, &p->cells_d, ngrid) || dalloc(p, &p->tlcell_d, ngrid)) return bail(-1);
+  }
   if (hipMemcpyAsync(p->rays_d, rays, sizeof(cpr_ray) * cfg->nrays, hipMemcpyHostToDevice, p->stream) ||
       hipMemcpyAsync(p->envs_d, envs, sizeof(cpr_env) * cfg->nrays, hipMemcpyHostToDevice, p->stream) ||
       hipMemcpyAsync(p->env_d, env_data, sizeof(double) * env_len, hipMemcpyHostToDevice, p->stream) ||
@@ -321,6 +331,11 @@ int cpr_selfs(cpr_plan *p) {
     a.u = dcol[4];
     a.cp = dcol[5];
     a.dp = dcol[6];
+  }
+  if (p->cells_d) {
+    const size_t ngrid = static_cast<size_t>(std::max(p->cfg.nvr_max, 1)) * p->ld *
+                         static_cast<size_t>(std::max(p->cfg.nvz_max, 1));
+    HIPCHK(hipMemsetAsync(p->cells_d, 0, ngrid * sizeof(float), p->stream));
   }
   const int block = 64;
   const int grid = (p->nrays + block - 1) / block;
@@ -474,6 +489,25 @@ int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double 
           cpg[2 * ((b * nvz + i) * nvr + t) + 1] = buf[2 * ((t * nvz + i) * ld + b) + 1];
         }
   }
+  return 0;
+}
+
+int cpr_read_cells(cpr_plan *p, double *tlc) {
+  if (!p || !tlc) return fail("null argument");
+  if (!p->cells_d) return fail("plan was created without compute_cells");
+  if (p->cfg.nvr_max > 0) {
+    const int block = 64;
+    hipLaunchKernelGGL(k_tl_post, dim3((p->nrays + block - 1) / block, p->cfg.nvr_max), dim3(block), 0, p->stream,
+                       p->args());
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(p->stream));
+  const int64_t ld = p->ld, nvr = p->cfg.nvr_max, nvz = p->cfg.nvz_max, n = p->nrays;
+  std::vector<double> buf(static_cast<size_t>(nvr * nvz * ld));
+  HIPCHK(hipMemcpy(buf.data(), p->tlcell_d, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int64_t b = 0; b < n; ++b)
+    for (int64_t i = 0; i < nvz; ++i)
+      for (int64_t t = 0; t < nvr; ++t) tlc[(b * nvz + i) * nvr + t] = buf[(t * nvz + i) * ld + b];
   return 0;
 }
 

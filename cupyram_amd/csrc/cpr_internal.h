@@ -50,6 +50,10 @@ struct MarchArgs {
   // tlg/cpg [nvr_max][nvz_max][ld]
   double *tll, *tlg;
   double *cpl, *cpg;  // complex128 interleaved
+  // cell-mean intensity: sums of |f3 u|^2 / r per output cell
+  // [nvr_max][nvz_max][ld] (FP32) and their TL (FP64); null unless requested
+  float *cells;
+  double *tlcell;
   // team variant
   const int32_t *team_rays;  // rays handled by this launch (team variant)
   int team_nrays;

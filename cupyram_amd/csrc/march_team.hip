@@ -340,6 +340,34 @@ __device__ inline double f3_from_cache(const MedCache<M> &mc, int k, int g0, int
   return (g0 + k <= iz) ? mc.w[k + 1][2] : mc.b[k + 1][2];
 }
 
+// Cell-mean intensity: adds |f3 u|^2 * rinv of the lane's rows to the output
+// cells of range bin `bin`, depth bin j holding rows j*ndz+1 .. (j+1)*ndz.
+// A lane's rows span consecutive depth bins; each partial sum is flushed
+// with one atomic add when its bin ends (lanes share bins).
+template <class R, int M>
+__device__ void add_cells(const MarchArgs &a, const cpr_ray &P, int b, int bin, const cx<R> (&u)[M],
+                          const MedCache<M> &mc, int g0, int mv, int iz, double rinv) {
+  float *col = a.cells + static_cast<int64_t>(bin) * a.nvz_max * a.ld + b;
+  const int ndz = P.ndz;
+  int j = (g0 - 1) / ndz;
+  int left = ndz - (g0 - 1) % ndz;
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    if (k < mv) {
+      const double f3 = f3_from_cache<M>(mc, k, g0, iz);
+      const double i2 = static_cast<double>(u[k].re) * u[k].re + static_cast<double>(u[k].im) * u[k].im;
+      acc += static_cast<float>(f3 * f3 * rinv * i2);
+    }
+    if (--left == 0 || k == M - 1) {
+      if (j < P.nvz && acc != 0.f) atomicAdd(col + static_cast<int64_t>(j) * a.ld, acc);
+      acc = 0.f;
+      ++j;
+      left = ndz;
+    }
+  }
+}
+
 // Re-form the coefficients of lane rows klo..khi (clamped to the lane).
 template <class R, int M, bool DS>
 __device__ void rebuild_rows(Coefs<R, M, DS> &C, const MedCache<M> &mc, int g0, int iz, double cfact, int klo,
@@ -507,6 +535,10 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
 #pragma unroll
       for (int k = 0; k < M - 1; ++k) L.u[k] = cfms(cfms(L.u[k], X.w(k), yl), X.c(k), yc);
       L.u[M - 1] = yc;
+    }
+    if (a.cells) {
+      const int bin = st.tlc + 1;
+      if (bin < P.nvr) add_cells<R, M>(a, P, b, bin, L.u, mc, g0, mv, st.iz, 1.0 / ((step + 2) * P.dr));
     }
     // output (outpt.py:11-58): complex pressure only; TL is formed by a
     // post-pass over the whole output array

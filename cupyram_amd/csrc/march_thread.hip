@@ -102,6 +102,27 @@ __device__ void factor_solve(const Col<R> &c, int nz, R k0, R cfact, cx<R> p1, c
   }
 }
 
+// Cell-mean intensity: adds |f3 u|^2 / rf of the field at range rf to the
+// output cells of range bin tlc+1 (the ndr fields that end at the next
+// output), depth bin j holding rows j*ndz+1 .. (j+1)*ndz.  The lane owns the
+// ray, so the sums are plain read-modify-writes.
+template <class R>
+__device__ void add_cells(const MarchArgs &a, const cpr_ray &P, const RayState &st, const Col<R> &c, double rf) {
+  const int bin = st.tlc + 1;
+  if (!a.cells || bin >= P.nvr) return;
+  float *col = a.cells + static_cast<int64_t>(bin) * a.nvz_max * a.ld + c.b;
+  const double rinv = 1.0 / rf;
+  for (int j = 0; j < P.nvz; ++j) {
+    double acc = 0.0;
+    for (int g = j * P.ndz + 1; g <= (j + 1) * P.ndz; ++g) {
+      const cx<R> v = c.U(g);
+      const double f3 = static_cast<double>(c.F3(g));
+      acc += f3 * f3 * (static_cast<double>(v.re) * v.re + static_cast<double>(v.im) * v.im);
+    }
+    col[static_cast<int64_t>(j) * a.ld] += static_cast<float>(acc * rinv);
+  }
+}
+
 // Complex pressure at the receiver and on the output grid (outpt.py:39-53);
 // the TL of every stored pressure is formed afterwards by k_tl_post.
 template <class R>
@@ -171,6 +192,7 @@ __global__ void k_selfs(MarchArgs a) {
   const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + 1) * 2 * P.np * 2;
   for (int j = 0; j < P.np; ++j)
     factor_solve<double>(c, P.nz, P.k0, cfact, pade_coef<double>(pd, j), pade_coef<double>(pd, P.np + j));
+  add_cells<double>(a, P, st, c, P.dr);
   outpt<double>(a, P, st, c);
   a.state[b] = st;
 }
@@ -214,6 +236,7 @@ __global__ void __launch_bounds__(64) k_march_thread(MarchArgs a, int s0, int s1
     const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
     for (int j = 0; j < P.np; ++j)
       factor_solve<R>(c, P.nz, k0, cfact, pade_coef<R>(pd, j), pade_coef<R>(pd, P.np + j));
+    add_cells<R>(a, P, st, c, (step + 2) * P.dr);
     outpt<R>(a, P, st, c);
   }
   a.state[b] = st;
@@ -235,6 +258,13 @@ __global__ void k_tl_post(MarchArgs a) {
   const double r = (t + 1) * P.ndr * P.dr;
   const int64_t o = static_cast<int64_t>(t) * a.ld + b;
   a.tll[o] = tl_of(a.cpl[2 * o], a.cpl[2 * o + 1], r);
+  if (a.cells) {
+    const double inv_n = 1.0 / (static_cast<double>(P.ndr) * P.ndz);
+    for (int i = 0; i < P.nvz; ++i) {
+      const int64_t og = (static_cast<int64_t>(t) * a.nvz_max + i) * a.ld + b;
+      a.tlcell[og] = -10.0 * log10(static_cast<double>(a.cells[og]) * inv_n + 1e-40);
+    }
+  }
   if (a.grids) {
     for (int i = 0; i < P.nvz; ++i) {
       const int64_t og = (static_cast<int64_t>(t) * a.nvz_max + i) * a.ld + b;

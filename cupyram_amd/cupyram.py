@@ -27,6 +27,10 @@ Extensions (keyword arguments the reference does not have):
                            wavelength of the absorbing layer is c0/f per ray)
   steps_per_launch=k       march in launches of k range steps captured in a
                            hipGraph (0: one launch for the whole march)
+  compute_cells=True       adds 'TL Cells' (shaped like 'TL Grid'): the TL of
+                           the mean of |p|^2/r over each output cell, ranges
+                           (i ndr, (i+1) ndr] dr x rows (j ndz, (j+1) ndz] --
+                           the coarse TL the learned surrogate is trained on
 """
 from __future__ import annotations
 
@@ -106,6 +110,7 @@ class CuPyRAM:
     def __init__(self, freq, zs, zr, z_ss, rp_ss, cw, z_sb, rp_sb, cb, rhob, attn, rbzb, **kwargs):
         self._batch_size = int(kwargs.get("batch_size", 1))
         self._compute_grids = bool(kwargs.get("compute_grids", True))
+        self._compute_cells = bool(kwargs.get("compute_cells", False))
         self._max_workers = kwargs.get("max_workers", 8)
         self._precision = {"c64": 64, "complex64": 64, 64: 64, "c128": 128, "complex128": 128,
                            128: 128}[kwargs.get("precision", "c128")]
@@ -371,7 +376,8 @@ class CuPyRAM:
         cfg.variant = _abi.VARIANTS[variant or self._variant]
         cfg.compute_grids = 1 if self._compute_grids else 0
         cfg.nvr_max = int(self._rays["nvr"][idx].max())
-        cfg.nvz_max = int(self._rays["nvz"][idx].max()) if self._compute_grids else 0
+        cfg.compute_cells = 1 if self._compute_cells else 0
+        cfg.nvz_max = int(self._rays["nvz"][idx].max()) if (self._compute_grids or self._compute_cells) else 0
         shape = shape or self._team_shape
         if shape:
             cfg.team_rows, cfg.team_lanes = shape
@@ -417,6 +423,7 @@ class CuPyRAM:
         cpl = numpy.zeros((nc, self._nvr), numpy.complex128)
         tlg = numpy.zeros((nc, self._nvz, self._nvr)) if self._compute_grids else None
         cpg = numpy.zeros((nc, self._nvz, self._nvr), numpy.complex128) if self._compute_grids else None
+        tlc = numpy.full((nc, self._nvz, self._nvr), numpy.nan) if self._compute_cells else None
         plans = []
         try:
             for idx, shape, variant in buckets:
@@ -441,6 +448,11 @@ class CuPyRAM:
                 tll[idx, :nv], cpl[idx, :nv] = a, b
                 if self._compute_grids:
                     tlg[idx, :g.shape[1], :nv], cpg[idx, :g.shape[1], :nv] = g, h
+                if self._compute_cells:
+                    cells = plan.read_cells()
+                    for k, c in enumerate(idx):
+                        nr_c, nz_c = int(self._rays["nvr"][c]), int(self._rays["nvz"][c])
+                        tlc[c, :nz_c, :nr_c] = cells[k, :nz_c, :nr_c]
                 self.plan_variants.append((len(idx), plan.describe(0)))
             self.plan_variant = self.plan_variants[0][1]
         finally:
@@ -456,6 +468,11 @@ class CuPyRAM:
                 cpl[c, nvr_c[c]:] = numpy.nan
         self.tll, self.cpl, self.tlg, self.cpg = tll, cpl, tlg, cpg
         ne, nf = self._n_env, self._n_freq
+        self.tlc = None
+        if self._compute_cells:
+            # same [env, freq, depth, range] reshaping and squeezing as the TL Grid
+            tlc = tlc.reshape(ne, nf, tlc.shape[1], tlc.shape[2])
+            self.tlc = tlc[0, 0] if (nf == 1 and ne == 1) else (tlc[:, 0] if nf == 1 else (tlc[0] if ne == 1 else tlc))
         self.tll = self.tll.reshape(ne, nf, -1)
         self.cpl = self.cpl.reshape(ne, nf, -1)
         if self._compute_grids:
@@ -481,4 +498,5 @@ class CuPyRAM:
                 "TL Line": self.tll,
                 "CP Grid": self.cpg,
                 "CP Line": self.cpl,
-                "c0": self._c0}
+                "c0": self._c0,
+                **({"TL Cells": self.tlc} if self._compute_cells else {})}

@@ -1,7 +1,8 @@
 """PE-generated training and query sets of the surrogate.
 
 Two query domains, each with a seeded generator and the march setup whose
-TL Grid lands exactly on the domain's output grid:
+output grid (the cells of 'TL Cells', or the points of 'TL Grid') is
+exactly the domain's output grid:
 
 deep_munk   the 10,000-ray Munk benchmark batch's generator (5000 m Munk
             water, 16 bathymetry knots on 0-50 km from U[3500, 5000] m,
@@ -78,7 +79,8 @@ def _bathy_table(dom, q, i):
 
 def march_kwargs(dom: Domain, q: Queries, precision="c64"):
     """CuPyRAM keyword arguments that march every query of q onto the
-    domain's output grid (TL Grid [n, nzp, nr])."""
+    domain's output grid: 'TL Cells' [n, nzp, nr] (target "cells") or
+    'TL Grid' (target "points")."""
     n = len(q)
     zk = np.linspace(0.0, dom.ssp_depth, N_SSP)
     if dom.name == "deep_munk":
@@ -116,5 +118,5 @@ def march_kwargs(dom: Domain, q: Queries, precision="c64"):
     kw.update(freq=q.freq, freq_per_ray=True, zs=q.zs, zr=float(min(50.0, dom.ssp_depth)),
               z_ss=zk, rp_ss=np.array([0.0]), cw=[c.reshape(-1, 1) for c in q.ssp],
               rbzb=[_bathy_table(dom, q, i) for i in range(n)], np=6, batch_size=n,
-              compute_grids=True, precision=precision)
+              compute_grids=dom.target == "points", compute_cells=dom.target == "cells", precision=precision)
     return kw

@@ -6,9 +6,15 @@ A query is the 101-number description of one TL field:
     bottom  [3]   three bottom parameters of the domain (see Domain.bottom_names)
     freq          source frequency (Hz)
     zs            source depth (m)
-and the answer is TL (dB) on the output grid of the domain: nr ranges
-r_j = (j+1) rmax/nr and nzp depths z_i = (i+1) zplot/nzp (the TL Grid of a
-march with ndr, ndz chosen so that its output points are exactly these).
+and the answer is the coarse TL (dB) on the output grid of the domain: nr
+range bins ending at r_j = (j+1) rmax/nr and nzp depth bins ending at
+z_i = (i+1) zplot/nzp, each cell holding -10 log10 of the mean of |p|^2/r
+over the march's fields and rows inside it (the 'TL Cells' of a march whose
+ndr, ndz make its output grid exactly these bins).  Point samples of the
+coherent field (the TL Grid) fluctuate like a Rayleigh amplitude from one
+point to the next, about 5.6 dB in standard deviation, which no smooth
+predictor follows; the cell means keep the propagation physics (mode
+stripping, convergence zones, bottom loss) without that interference noise.
 
 The encoding gives every (range bin, depth point) the 101 normalised query
 numbers plus 7 local ones (range, depth, local water depth, height above
@@ -44,6 +50,7 @@ class Domain:
     tl_hi: float = 140.0
     tl_mean: float = 80.0    # normalised target = (TL - tl_mean) / tl_std
     tl_std: float = 20.0
+    target: str = "cells"    # "cells" (cell-mean TL) or "points" (TL Grid)
 
     def ranges(self):
         return (np.arange(self.nr) + 1.0) * (self.rmax / self.nr)

@@ -35,8 +35,9 @@ def _log(msg):
 
 
 def pe_fields(dom: Domain, q: Queries, precision="c64", chunk=10000, log=_log):
-    """TL (dB) of every query on the domain grid, float32 [n, nr, nzp] on the
-    GPU, and the march time (s) and cell-steps."""
+    """TL (dB) of every query on the domain grid (cell means or point samples
+    per the domain's target), float32 [n, nr, nzp] on the GPU, and the march
+    time (s) and cell-steps."""
     n = len(q)
     out = torch.empty(n, dom.nr, dom.nzp, dtype=torch.float32, device="cuda")
     t_march, cells = 0.0, 0.0
@@ -48,7 +49,8 @@ def pe_fields(dom: Domain, q: Queries, precision="c64", chunk=10000, log=_log):
         t_march += time.perf_counter() - t0
         r = m._rays
         cells += float(np.sum(r["nz"].astype(np.float64) * r["nsteps"]))
-        tlg = np.asarray(res["TL Grid"]).reshape(len(idx), dom.nzp, dom.nr)
+        key = "TL Cells" if dom.target == "cells" else "TL Grid"
+        tlg = np.asarray(res[key]).reshape(len(idx), dom.nzp, dom.nr)
         out[a:a + len(idx)] = torch.from_numpy(np.ascontiguousarray(tlg.transpose(0, 2, 1))).float().cuda()
         if log:
             log(f"  PE {precision} {dom.name}: {a + len(idx)}/{n} fields ({t_march:.1f} s)")

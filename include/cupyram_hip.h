@@ -74,6 +74,7 @@ typedef struct {
   int32_t nvr_max, nvz_max;
   int32_t team_rows;     /* rows per lane of the team variant (0 = auto) */
   int32_t team_lanes;    /* lanes per ray of the team variant (0 = auto) */
+  int32_t compute_cells; /* 1: cell-mean TL on the output grid (cpr_read_cells) */
 } cpr_config;
 
 typedef struct cpr_plan cpr_plan;
@@ -110,6 +111,14 @@ int cpr_march_time(cpr_plan *p, double *ms, int *nlaunch);
 /* Outputs, ray-major: tll [nrays][nvr_max] f64, cpl [nrays][nvr_max] c128,
  * tlg [nrays][nvz_max][nvr_max] f64, cpg likewise c128.  Any may be NULL. */
 int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double *cpg);
+/* Cell-mean TL, [nrays][nvz_max][nvr_max] f64 (plan created with
+ * compute_cells): cell (i, j) holds -10 log10 of the mean of |p|^2 / r over
+ * the ndr fields at ranges (i ndr, (i+1) ndr] dr and the ndz rows
+ * (j ndz, (j+1) ndz] of each, p = f3 u as in the TL grid.  No counterpart in
+ * the reference: this is the coarse TL field the learned surrogate is
+ * trained on (the point samples of the TL grid carry the interference
+ * pattern, which no smooth predictor reproduces). */
+int cpr_read_cells(cpr_plan *p, double *tlc);
 /* Field u after the last march step, [nrays][nz_max+2] complex128. */
 int cpr_read_field(cpr_plan *p, double *u, int nz_max);
 /* Bytes of device memory held by the plan. */

@@ -24,12 +24,17 @@
  *   ray_outpt     cupyram/outpt.py:11-58 (TL line + TL grid; the CP grid is
  *                 written too -- the reference's kernel drops it, outpt.py:50)
  *   orc_run       cupyram/cupyram.py:273-380 + setup :528-726
+ *   orc_run_cells the same march with cell-mean TL on the output grid (no
+ *                 reference counterpart: the coarse TL of the surrogate)
  *
  * Parity pins (tests/test_oracle.py): RAM v1.5's tl_ref.line (mean |dTL| <=
  * 0.01 dB), Pade closed forms (Taylor series by series composition, stability
  * zeros), numpy.interp for profiles, brute-force bathymetry indices,
  * identities of the factor solve (identity map, inverse factor, linearity,
  * independence of the meeting row), and a textbook Pekeris normal-mode sum.
+ * Cell means are pinned against the TL grid of the same march: with
+ * ndr = ndz = 1 every cell is one grid point, and a coarse cell equals the
+ * intensity mean of the fine grid points it covers.
  */
 #include <complex.h>
 #include <math.h>
@@ -261,6 +266,7 @@ typedef struct {
   /* outputs */
   double *tll, *tlg;
   cplx *cpl, *cpg;
+  double *cells; /* [nvz][nvr] sums of |f3 u|^2 / r per output cell, or NULL */
   int *iz_hist;
 } ray_state;
 
@@ -416,6 +422,19 @@ static void ray_solve(ray_state *s, int np_) {
 static void ray_outpt(ray_state *s) {
   const orc_ray *in = s->in;
   const double eps = 1e-20;
+  /* cell means: the field at range r belongs to range bin tlc+1 (the ndr
+   * fields ending at the next output); depth bin i holds rows
+   * i*ndz+1 .. (i+1)*ndz */
+  if (s->cells && s->tlc + 1 < s->nvr) {
+    for (int i = 0; i < s->nvz; ++i) {
+      double acc = 0.0;
+      for (int g = i * in->ndz + 1; g <= (i + 1) * in->ndz; ++g) {
+        const cplx p = s->u[g] * s->f3[g];
+        acc += creal(p) * creal(p) + cimag(p) * cimag(p);
+      }
+      s->cells[(size_t)i * s->nvr + s->tlc + 1] += acc / s->r;
+    }
+  }
   s->mdr += 1;
   if (s->mdr != in->ndr) return;
   s->mdr = 0;
@@ -512,8 +531,8 @@ int orc_dims(const orc_ray *in, int *nz, int *nvr, int *nvz, int *nsteps) {
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
-int orc_run(const orc_ray *in, double *tll, double complex *cpl, double *tlg, double complex *cpg,
-            int *iz_hist) {
+static int run_impl(const orc_ray *in, double *tll, double complex *cpl, double *tlg, double complex *cpg,
+                    int *iz_hist, double *tlcell) {
   ray_state s;
   memset(&s, 0, sizeof(s));
   s.in = in;
@@ -580,6 +599,8 @@ int orc_run(const orc_ray *in, double *tll, double complex *cpl, double *tlg, do
   s.cpl = (cplx *)cpl;
   s.tlg = tlg;
   s.cpg = (cplx *)cpg;
+  s.cells = tlcell;
+  if (tlcell) memset(tlcell, 0, sizeof(double) * (size_t)s.nvz * s.nvr);
   s.mdr = 0;
   s.tlc = -1;
   int rc = 0;
@@ -613,6 +634,10 @@ int orc_run(const orc_ray *in, double *tll, double complex *cpl, double *tlg, do
     s.r = (rn + 2) * in->dr;
     ray_outpt(&s);
   }
+  if (tlcell) {
+    const double inv_n = 1.0 / ((double)in->ndr * in->ndz);
+    for (size_t k = 0; k < (size_t)s.nvz * s.nvr; ++k) tlcell[k] = -10.0 * log10(tlcell[k] * inv_n + 1e-40);
+  }
 done:
   free(s.rbzb);
   free(s.z_sb);
@@ -634,6 +659,15 @@ done:
   free(s.s2);
   free(s.s3);
   return rc;
+}
+
+int orc_run(const orc_ray *in, double *tll, double complex *cpl, double *tlg, double complex *cpg,
+            int *iz_hist) {
+  return run_impl(in, tll, cpl, tlg, cpg, iz_hist, NULL);
+}
+
+int orc_run_cells(const orc_ray *in, double *tll, double *tlcell) {
+  return run_impl(in, tll, NULL, NULL, NULL, NULL, tlcell);
 }
 
 int orc_run_batch(const orc_ray *rays, int nrays, double *const *tll, int nthreads) {

@@ -30,6 +30,10 @@ int orc_pade(double freq, double c0, int np_, int ns, double dr, int ip, double 
 int orc_dims(const orc_ray *in, int *nz, int *nvr, int *nvz, int *nsteps);
 int orc_run(const orc_ray *in, double *tll, double complex *cpl, double *tlg, double complex *cpg,
             int *iz_hist);
+/* TL line and cell-mean TL [nvz][nvr]: cell (i, j) = -10 log10 of the mean
+ * of |f3 u|^2 / r over the fields at ranges (i ndr, (i+1) ndr] dr and rows
+ * (j ndz, (j+1) ndz] (the build's coarse TL; no reference counterpart). */
+int orc_run_cells(const orc_ray *in, double *tll, double *tlcell);
 int orc_run_batch(const orc_ray *rays, int nrays, double *const *tll, int nthreads);
 int orc_profiles(const orc_ray *in, int ss_ind, int sb_ind, double *alpw, double *alpb, double *rhob,
                  double complex *ksqw, double complex *ksqb);

@@ -55,6 +55,7 @@ def lib():
         L.orc_dims.argtypes = [C.POINTER(_Ray)] + [C.POINTER(C.c_int)] * 4
         L.orc_run.argtypes = [C.POINTER(_Ray), dp, dp, dp, dp, C.POINTER(C.c_int)]
         L.orc_run_batch.argtypes = [C.POINTER(_Ray), C.c_int, C.POINTER(dp), C.c_int]
+        L.orc_run_cells.argtypes = [C.POINTER(_Ray), dp, dp]
         L.orc_profiles.argtypes = [C.POINTER(_Ray), C.c_int, C.c_int, dp, dp, dp, dp, dp]
         L.orc_apply_factors.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, dp,
                                         dp, dp, dp, dp, dp, dp]
@@ -175,6 +176,20 @@ def run(ray: OracleRay, grids=True):
             "Depths": np.arange(1, nvz + 1) * ray.dz * ray.ndz,
             "TL Line": tll, "CP Line": cpl, "TL Grid": tlg, "CP Grid": cpg,
             "iz": izh[:nsteps], "nz": nz}
+
+
+def run_cells(ray: OracleRay):
+    """TL line and cell-mean TL [nvz, nvr] (same layout as the TL Grid)."""
+    nz, nvr, nvz, nsteps = ray.dims()
+    s = ray.struct()
+    tll = np.zeros(nvr)
+    tlc = np.zeros((nvz, nvr))
+    rc = lib().orc_run_cells(C.byref(s), _dp(tll), _dp(tlc))
+    if rc == -1:
+        raise ValueError("Laguerre method not converging")
+    if rc:
+        raise ValueError(f"orc_run_cells failed ({rc})")
+    return {"TL Line": tll, "TL Cells": tlc}
 
 
 def run_batch_tl(rays, nthreads=None):

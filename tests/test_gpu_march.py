@@ -189,3 +189,47 @@ def test_bucketing_independent_of_batch_composition(require_gpu):
         a, b = full["TL Line"][i], part["TL Line"][j]
         n = np.sum(np.isfinite(b))
         assert np.array_equal(a[:n], b[:n])
+
+
+def _cells_vs_oracle(kw, precision, variant, **extra):
+    m, res = _run(kw, precision=precision, variant=variant, compute_cells=True, **extra)
+    nc = m._total_batch
+    tlc = np.asarray(res["TL Cells"]).reshape(nc, *np.shape(res["TL Cells"])[-2:])
+    worst = 0.0
+    for e in range(m._n_env):
+        o = O.run_cells(envs.oracle_for(m, e, 0))["TL Cells"]
+        g = tlc[e, :o.shape[0], :o.shape[1]]
+        s = envs.tl_stats(np.clip(g, 0, 200), np.clip(o, 0, 200))
+        worst = max(worst, s["rms"])
+    return worst
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-6), ("c64", 0.05)])
+def test_cells_match_oracle(require_gpu, variant, precision, tol):
+    """Cell-mean TL (ranges (i ndr, (i+1) ndr] dr x rows (j ndz, (j+1) ndz])
+    on the sloping wedge, team lanes sharing depth bins (atomic flushes) and
+    the thread march, vs the oracle's cell means."""
+    kw = envs.wedge(rmax=4000.0)
+    kw.update(zmplt=150.0, ndr=3, ndz=4)
+    worst = _cells_vs_oracle(kw, precision, variant)
+    print(precision, variant, "cells worst rms", worst)
+    assert worst <= tol
+
+
+@pytest.mark.parametrize("shape", [(4, 64), (12, 64), (24, 64), (48, 64), (8, 128)])
+def test_cells_every_lane_layout(require_gpu, shape):
+    """Depth bins of 5 rows straddle lane boundaries differently for each
+    rows-per-lane count (register, streamed and LDS teams)."""
+    kw = envs.wedge(rmax=2000.0)
+    kw.update(zmplt=150.0, ndr=2, ndz=5)
+    worst = _cells_vs_oracle(kw, "c64", "team", team_shape=shape)
+    assert worst <= 0.05, (shape, worst)
+
+
+def test_cells_uneven_batch(require_gpu):
+    """Per-ray grids in several buckets: each ray's cells vs its oracle run."""
+    kw = _uneven(n=6, rmax=2000.0)
+    kw.update(ndr=2, ndz=3, zmplt=90.0)
+    worst = _cells_vs_oracle(kw, "c64", "auto")
+    assert worst <= 0.05, worst

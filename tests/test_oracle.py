@@ -276,3 +276,40 @@ def test_pekeris_normal_modes():
     # a wrong bottom density would move the field by several dB
     tl_bad = pekeris_tl(r, 50.0, 100.0, 1500.0, 1700.0, 1.0, 2.5, 0.5, 25.0, 30.0)
     assert np.sqrt(np.mean((out["TL Line"][sel] - tl_bad[sel]) ** 2)) > 2 * rms
+
+
+# -- cell-mean TL (the surrogate's coarse TL) ------------------------------------
+
+def _wedge_ray(**over):
+    kw = envs.wedge(rmax=3000.0)
+    kw.update(zmplt=150.0)
+    kw.update(over)
+    return O.resolve_ray(**kw)
+
+
+def test_cells_of_one_point_are_the_tl_grid():
+    """ndr = ndz = 1: every cell holds exactly one grid point, so the cell TL
+    is the TL grid (a wrong range bin or depth row shifts the pattern)."""
+    ray = _wedge_ray()
+    grid = O.run(ray)["TL Grid"]
+    cells = O.run_cells(ray)["TL Cells"]
+    assert cells.shape == grid.shape
+    ok = grid < 200.0     # away from exact zeros of the field
+    np.testing.assert_allclose(cells[ok], grid[ok], atol=1e-9)
+
+
+@pytest.mark.parametrize("ndr,ndz", [(2, 3), (5, 1), (1, 4)])
+def test_cells_are_intensity_means_of_the_fine_grid(ndr, ndz):
+    """A coarse cell is the mean of |p|^2 / r = 10^(-TL/10) over the ndr x ndz
+    fine grid points it covers: ranges (i ndr, (i+1) ndr] dr, rows
+    (j ndz, (j+1) ndz] dz."""
+    fine = O.run(_wedge_ray())["TL Grid"]                    # [rows, ranges] at every dz, dr
+    coarse = O.run_cells(_wedge_ray(ndr=ndr, ndz=ndz))["TL Cells"]
+    nvz, nvr = coarse.shape
+    inten = 10.0 ** (-fine / 10.0)
+    expect = np.empty_like(coarse)
+    for j in range(nvz):
+        for i in range(nvr):
+            blk = inten[j * ndz:(j + 1) * ndz, i * ndr:(i + 1) * ndr]
+            expect[j, i] = -10.0 * np.log10(blk.mean())
+    np.testing.assert_allclose(coarse, expect, atol=1e-8)
