@@ -35,7 +35,7 @@ METRIC = "Mcell-steps/s (nz×range steps) c64 vs c128 at ≤0.5 dB TL err; HBM G
 # HBM bytes per cell-step of the complex64 team march on the 10,000-ray Munk
 # batch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy
 # (factors 2.0 and 1.0); keyed by (precision, team rows, team lanes).
-MEASURED_TRAFFIC = {(64, 24, 256): {"fetch": 170.4, "write": 4.1,
+MEASURED_TRAFFIC = {(64, 24, 256): {"fetch": 171.2, "write": 1.78,
                                      "source": "profiles/team_march_munk10k.md"}}
 
 
@@ -169,26 +169,38 @@ def bench_uneven(args):
     return 0
 
 
-def bench_wedge(args):
-    """Surrogate-first answering of the wedge queries with complex64 PE
-    fallback, beside the plain complex64 PE march of every query; TL error of
-    both against the complex128 march on the domain grid (TL clipped to
+ROUTER_WORKLOADS = {
+    # domain, default query count, query seed, workload text
+    "wedge": ("wedge", 100000, 5, "wedge_queries: {n} queries, 200->20 m over 10 km, sediment 5-30 m over "
+                                  "basement, freq logU 50-1000 Hz, np 6, output 200 x 64 cell-mean TL"),
+    "deep": ("deep_munk", 20000, 7, "deep_queries: {n} queries, Munk 5000 m perturbed at 32 knots, 16-knot "
+                                   "bathymetry 3500-5000 m over 50 km, freq logU 80-125 Hz, np 6, output 200 x 64 "
+                                   "cell-mean TL"),
+}
+
+
+def bench_router(args):
+    """Surrogate-first answering of a query set with complex64 PE fallback,
+    beside the plain complex64 PE march of every query; TL error of both
+    against the complex128 march on the domain grid (cell-mean TL clipped to
     20-140 dB)."""
     import torch
     from cupyram_amd.surrogate import dataset as D
     from cupyram_amd.surrogate.router import Router
     from cupyram_amd.surrogate.train import field_rms, load_ensemble, pe_fields, summary
     log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
-    dom = D.WEDGE
-    n = args.rays if args.rays != 10000 else 100000
-    q = D.wedge_queries(n, 5)
+    dname, n_default, qseed, text = ROUTER_WORKLOADS[args.workload]
+    dom = D.DOMAINS[dname]
+    n = args.rays if args.rays != 10000 else n_default
+    gen = D.deep_queries if dname == "deep_munk" else D.wedge_queries
+    q = gen(n, qseed)
     members = load_ensemble(dom.name)
-    log(f"wedge: {n} queries, {len(members)} ensemble members")
-    router = Router(dom, members, threshold_db=3.0, spot_fraction=0.01, seed=5, precision="c64", log=None)
+    log(f"{dname}: {n} queries, {len(members)} ensemble members")
+    router = Router(dom, members, threshold_db=3.0, spot_fraction=0.01, seed=qseed, precision="c64", log=None)
     router.ensemble(q.take(np.arange(min(n, 256))))          # warm-up (buffers, kernels)
     torch.cuda.synchronize()
     res = router.run(q)
-    log(f"wedge: routed {res.frac_routed:.3f} in {res.seconds['total']:.1f} s")
+    log(f"{dname}: routed {res.frac_routed:.3f} in {res.seconds['total']:.1f} s")
     t0 = time.perf_counter()
     tl64, t64, cells = pe_fields(dom, q, "c64", chunk=20000, log=log)
     torch.cuda.synchronize()
@@ -199,11 +211,10 @@ def bench_wedge(args):
     acc = ~res.routed
     sur = summary(field_rms(dom, res.tl[torch.as_tensor(np.nonzero(acc)[0], device="cuda")],
                             tl128[torch.as_tensor(np.nonzero(acc)[0], device="cuda")])) if acc.any() else None
-    line = {"metric": "wedge queries/s, surrogate-first with complex64 PE fallback (TL error vs complex128)",
+    line = {"metric": f"{dname} queries/s, surrogate-first with complex64 PE fallback (TL error vs complex128)",
             "value": n / res.seconds["total"], "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
-            "data": "synthetic (seeded wedge queries, seed 5)",
-            "config": {"workload": f"wedge_queries: {n} queries, 200->20 m over 10 km, sediment 5-30 m over "
-                                   "basement, freq logU 50-1000 Hz, np 6, output 200 x 64",
+            "data": f"synthetic (seeded {dname} queries, seed {qseed})",
+            "config": {"workload": text.format(n=n),
                        "members": len(members), "threshold_db": 3.0, "spot_fraction": 0.01},
             "frac_routed_to_pe": res.frac_routed, "batch_fallback": res.batch_fallback,
             "seconds": res.seconds, "ensemble_spread_median_db": float(np.median(res.spread_db)),
@@ -287,7 +298,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=128)
     ap.add_argument("--cpu-rmax", type=float, default=5000.0)
-    ap.add_argument("--workload", default="munk", choices=["munk", "uneven", "wedge", "fno"])
+    ap.add_argument("--workload", default="munk", choices=["munk", "uneven", "wedge", "deep", "fno"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device-to-device (torch) so that HBM counters can be scaled")
     args = ap.parse_args(argv)
@@ -302,8 +313,8 @@ def main(argv=None):
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.workload == "wedge":
-        return bench_wedge(args)
+    if args.workload in ROUTER_WORKLOADS:
+        return bench_router(args)
     if args.workload == "fno":
         return bench_fno(args)
     if args.workload == "uneven":

@@ -18,8 +18,11 @@ stripping, convergence zones, bottom loss) without that interference noise.
 
 The encoding gives every (range bin, depth point) the 101 normalised query
 numbers plus 7 local ones (range, depth, local water depth, height above
-the seabed, offset from the source depth, a below-seabed flag, a constant),
-zero-padded to `features` columns; one network sample is one range bin.
+the seabed, offset from the source depth, a below-seabed flag, a constant)
+and 17 Fourier features of position (log range; sin and cos of pi k r/rmax
+and of pi k z/zplot for k = 1, 2, 4, 8), zero-padded to `features` columns;
+one network sample is one range bin.  The Fourier features let the first
+(linear) layer express structure along range and depth at several scales.
 """
 from __future__ import annotations
 
@@ -29,6 +32,9 @@ import numpy as np
 import torch
 
 N_SSP, N_BATHY, N_GLOBAL, N_LOCAL = 32, 64, 101, 7
+FOURIER_K = (1, 2, 4, 8)
+N_FOURIER = 1 + 4 * len(FOURIER_K)
+ENCODING = "v2"   # recorded with trained weights; v2 added the Fourier features
 
 
 @dataclasses.dataclass(frozen=True)
@@ -93,7 +99,7 @@ def global_features(dom: Domain, q: Queries, device="cuda"):
 def encode(dom: Domain, q: Queries, features: int = 128, device="cuda"):
     """Network input, bf16 [n*nr, nzp, features]."""
     n = len(q)
-    if features < N_GLOBAL + N_LOCAL:
+    if features < N_GLOBAL + N_LOCAL + N_FOURIER:
         raise ValueError("features too small")
     dev = torch.device(device)
     t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev)  # noqa: E731
@@ -116,6 +122,13 @@ def encode(dom: Domain, q: Queries, features: int = 128, device="cuda"):
     out[..., N_GLOBAL + 4] = z[None, None, :] - zsn[:, None, None]
     out[..., N_GLOBAL + 5] = (z[None, None, :] > zb[:, :, None]).float()
     out[..., N_GLOBAL + 6] = 1.0
+    o = N_GLOBAL + N_LOCAL
+    out[..., o] = torch.log(r)[None, :, None]
+    for i, kf in enumerate(FOURIER_K):
+        out[..., o + 1 + 4 * i] = torch.sin(np.pi * kf * r)[None, :, None]
+        out[..., o + 2 + 4 * i] = torch.cos(np.pi * kf * r)[None, :, None]
+        out[..., o + 3 + 4 * i] = torch.sin(np.pi * kf * z)[None, None, :]
+        out[..., o + 4 + 4 * i] = torch.cos(np.pi * kf * z)[None, None, :]
     return out.to(torch.bfloat16).reshape(n * dom.nr, dom.nzp, features)
 
 

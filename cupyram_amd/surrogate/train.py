@@ -23,7 +23,7 @@ import torch
 
 from ..cupyram import CuPyRAM
 from . import dataset as D
-from .encode import Domain, Queries, denormalise_tl, encode, global_features, normalise_tl
+from .encode import ENCODING, Domain, Queries, denormalise_tl, encode, global_features, normalise_tl
 from .fno import FNO, FNOConfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -163,6 +163,7 @@ def save_member(fno, dom, path):
         t = t.detach().cpu()
         d[n] = t.contiguous() if n.startswith(("b", "c")) else t.to(torch.bfloat16).contiguous()
     meta = {"domain": dom.name, "config": json.dumps(list(fno.cfg.__dict__.values())),
+            "encoding": ENCODING, "target": dom.target,
             "format": "FNO params; product weights bf16 (as used), biases f32"}
     save_file(d, path, metadata=meta)
 
@@ -172,6 +173,9 @@ def load_member(path, device="cuda"):
     from safetensors.torch import load_file
     with safe_open(path, "pt") as f:
         meta = f.metadata()
+    if meta.get("encoding") != ENCODING:
+        raise ValueError(f"{path}: weights trained on input encoding {meta.get('encoding')!r}, "
+                         f"this library encodes {ENCODING!r}; retrain")
     cfg = FNOConfig(*json.loads(meta["config"]))
     fno = FNO(cfg, seed=0, device=device)
     fno.load_state_dict({n: t.float() for n, t in load_file(path).items()})
@@ -191,6 +195,9 @@ def main(argv=None):
     ap.add_argument("--n-train", type=int, default=50000)
     ap.add_argument("--n-val", type=int, default=2000)
     ap.add_argument("--members", type=int, default=4)
+    ap.add_argument("--first-member", type=int, default=0,
+                    help="index of the first member trained here (members of one ensemble may be trained "
+                         "in separate runs on the same data)")
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -215,7 +222,7 @@ def main(argv=None):
            "pe_cell_steps": c_tr + c_va, "members": []}
     os.makedirs(args.out, exist_ok=True)
     preds = []
-    for m in range(args.members):
+    for m in range(args.first_member, args.first_member + args.members):
         fno, hist = train_member(dom, q_tr, tl_tr, epochs=args.epochs, batch=args.batch, lr=args.lr,
                                  seed=args.seed + 10 + m)
         save_member(fno, dom, os.path.join(args.out, f"{dom.name}_m{m}.safetensors"))

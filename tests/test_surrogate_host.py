@@ -5,7 +5,7 @@ import pytest
 
 from cupyram_amd import CuPyRAM
 from cupyram_amd.surrogate import dataset as D
-from cupyram_amd.surrogate.encode import N_GLOBAL, encode
+from cupyram_amd.surrogate.encode import FOURIER_K, N_FOURIER, N_GLOBAL, N_LOCAL, encode
 from cupyram_amd.surrogate.fno import dft_matrices
 
 
@@ -87,7 +87,12 @@ def test_encode_local_features():
     np.testing.assert_array_equal(x[0, :, :, N_GLOBAL + 5][clear], below[clear])
     np.testing.assert_allclose(x[0, :, 0, N_GLOBAL + 0], r / dom.rmax, rtol=1e-2)
     np.testing.assert_allclose(x[0, 0, :, N_GLOBAL + 1], z / dom.zplot, rtol=1e-2)
-    assert np.all(x[..., N_GLOBAL + 7:] == 0)
+    o = N_GLOBAL + N_LOCAL
+    np.testing.assert_allclose(x[0, :, 0, o], np.log(r / dom.rmax), rtol=1e-2, atol=1e-2)
+    for i, kf in enumerate(FOURIER_K):
+        np.testing.assert_allclose(x[0, :, 0, o + 1 + 4 * i], np.sin(np.pi * kf * r / dom.rmax), atol=1e-2)
+        np.testing.assert_allclose(x[0, 0, :, o + 4 + 4 * i], np.cos(np.pi * kf * z / dom.zplot), atol=1e-2)
+    assert np.all(x[..., o + N_FOURIER:] == 0)
     # global part is the same for every point of a query and differs between queries
     assert np.all(x[0, :, :, :N_GLOBAL] == x[0, :1, :1, :N_GLOBAL])
     assert not np.array_equal(x[0, 0, 0, :N_GLOBAL], x[1, 0, 0, :N_GLOBAL])
