@@ -104,6 +104,11 @@ _SYMBOLS = {
                       C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_void_p], C.c_int),
     "cpr_fno_fill": ([C.c_void_p, C.c_int64, C.c_float, C.c_void_p], C.c_int),
     "cpr_fno_cast_bf16": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "cpr_fno_cast_bf16_t": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_layer": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_project": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int,
+                         C.c_void_p], C.c_int),
 }
 
 

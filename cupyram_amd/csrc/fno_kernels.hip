@@ -325,6 +325,239 @@ __global__ void k_cast_bf16(const float *x, bf16 *y, int64_t n) {
   if (i < n) y[i] = static_cast<bf16>(x[i]);
 }
 
+// y[c][r] = bf16(x[r][c]) for an R x Cc matrix x (transposed cast)
+__global__ void k_cast_bf16_t(const float *x, bf16 *y, int R, int Cc) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<int64_t>(R) * Cc) return;
+  const int r = static_cast<int>(i / Cc), c = static_cast<int>(i % Cc);
+  y[static_cast<int64_t>(c) * R + r] = static_cast<bf16>(x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference kernels (Z = 64 depth points, C = 128 channels, K = 16
+// modes, F = 128 input features).  One block of four waves handles two depth
+// columns (128 rows); wave w owns column sb = w/2 of the block and output
+// channels 64*(w%2) .. +63.  MFMA operand fragments of
+// v_mfma_f32_32x32x16_bf16: A row = lane%32 with 8 consecutive k at
+// 8*(lane/32); B column = lane%32, same k; accumulator lane holds column
+// lane%32, rows 8*(r/4) + 4*(lane/32) + r%4 (r = 0..15).
+//
+//   k_fno_layer<SYN, ACT, ANA>:
+//     acc  = hin[rows] . W                         (W passed as WT[n][k])
+//     acc += Bsyn[z][k'] . M[s][k'][c]             (SYN: spectral branch,
+//            M the mixed modes [K][S][2C] with k' < K the real and k' >= K
+//            the imaginary parts; Bsyn = hi + lo bf16 split of the FP32
+//            synthesis matrix, so the product keeps ~16 mantissa bits)
+//     hout = ACT ? GELU(acc + bias) : acc + bias   (bf16)
+//     xh[k'][s][c] = sum_z Ban[k'][z] hout[s][z][c] (ANA: the next layer's
+//            truncated DFT, Ban = hi + lo split of the analysis matrix)
+//   k_fno_project: y[row] = GELU(h . P1 + c1) . P2 + c2 (f32)
+//
+// They replace, for inference, the sequence spec_synth -> gemm(GELU, addend)
+// -> spec_analysis of the layer kernels above: the FP32 spectral field y and
+// the separate transforms never reach HBM.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int FZ = 64, FC = 128, FK2 = 32;  // depth points, channels, 2 x modes
+constexpr int HT_LD = FZ + 8;               // padded z stride of the transposed tile
+
+__device__ inline bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+struct LayerArgs {
+  const bf16 *hin;   // [S][Z][C] (or the network input [S][Z][F], F = C)
+  const bf16 *wt;    // [C out][C in]
+  const float *bias; // [C]
+  const bf16 *mh;    // [K][S][2C] mixed modes (SYN)
+  const bf16 *bsyn;  // [2][Z][2K] hi, lo
+  const bf16 *ban;   // [2][2K][Z] hi, lo
+  bf16 *hout;        // [S][Z][C]
+  bf16 *xh;          // [K][S][2C] (ANA)
+  int S;
+};
+
+// acc[i][j] (+)= hin rows . W for the wave's 64 rows (2 tiles) x 64 channels (2 tiles)
+__device__ inline void pointwise_tiles(const bf16 *hrows, const bf16 *wt, int wn, int lane, f32x16 (&acc)[2][2]) {
+#pragma unroll
+  for (int kk = 0; kk < FC / 16; ++kk) {
+    const int k = kk * 16 + 8 * (lane >> 5);
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = ld8(hrows + (i * 32 + (lane & 31)) * FC + k);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = ld8(wt + (wn + j * 32 + (lane & 31)) * FC + k);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+}  // namespace
+
+template <bool SYN, bool ACT, bool ANA>
+__global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 ms[2][FK2][FC];    // mixed modes of the block's two columns
+  __shared__ __attribute__((aligned(16))) bf16 ht[2][FC][HT_LD];  // output tile, transposed (channel-major)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int sb = wave >> 1, wn = (wave & 1) * 64;
+  const int s0 = blockIdx.x * 2, s = s0 + sb;
+  const bool live = s < g.S;
+  if (SYN) {
+    // stage M[s][k'][c] for both columns: 2 x 32 x 128 bf16 in 16-byte pieces
+    for (int q = tid; q < 2 * FK2 * (FC / 8); q += 256) {
+      const int c8 = (q % (FC / 8)) * 8, kp = (q / (FC / 8)) % FK2, b2 = q / (FC / 8 * FK2);
+      const int ss = s0 + b2;
+      bf16x8 v;
+      if (ss < g.S) {
+        const int k = kp % (FK2 / 2), im = kp / (FK2 / 2);
+        v = ld8(g.mh + (static_cast<int64_t>(k) * g.S + ss) * 2 * FC + im * FC + c8);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = static_cast<bf16>(0.0f);
+      }
+      *reinterpret_cast<bf16x8 *>(&ms[b2][kp][c8]) = v;
+    }
+    __syncthreads();
+  }
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  if (live) pointwise_tiles(g.hin + static_cast<int64_t>(s) * FZ * FC, g.wt, wn, lane, acc);
+  if (SYN) {
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const bf16 *bs = g.bsyn + part * FZ * FK2;
+#pragma unroll
+      for (int ks = 0; ks < FK2 / 16; ++ks) {
+        const int kp = ks * 16 + 8 * (lane >> 5);
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = ld8(bs + (i * 32 + (lane & 31)) * FK2 + kp);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) b[j][e] = ms[sb][kp + e][wn + j * 32 + (lane & 31)];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: bias, activation, bf16 output (and its transposed copy for ANA)
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int n = wn + j * 32 + (lane & 31);
+    const float bn = g.bias ? g.bias[n] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        float v = acc[i][j][r] + bn;
+        if (ACT) v = gelu_f(v);
+        const bf16 hv = static_cast<bf16>(v);
+        if (live) g.hout[(static_cast<int64_t>(s) * FZ + z) * FC + n] = hv;
+        if (ANA) ht[sb][n][z] = live ? hv : static_cast<bf16>(0.0f);
+      }
+  }
+  if (ANA) {
+    __syncthreads();
+    // xh[k'][s][c] = sum_z Ban[k'][z] h[s][z][c]: 32 x 64 (k' x c) per wave
+    f32x16 a2[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) a2[j][r] = 0.0f;
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const bf16 *ba = g.ban + part * FK2 * FZ;
+#pragma unroll
+      for (int ks = 0; ks < FZ / 16; ++ks) {
+        const int z = ks * 16 + 8 * (lane >> 5);
+        const bf16x8 a = ld8(ba + (lane & 31) * FZ + z);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&ht[sb][wn + j * 32 + (lane & 31)][z]);
+          a2[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, a2[j], 0, 0, 0);
+        }
+      }
+    }
+    if (live) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int c = wn + j * 32 + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int kp = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const int k = kp % (FK2 / 2), im = kp / (FK2 / 2);
+          g.xh[(static_cast<int64_t>(k) * g.S + s) * 2 * FC + im * FC + c] = static_cast<bf16>(a2[j][r]);
+        }
+      }
+    }
+  }
+}
+
+// y[row] = GELU(h . P1 + c1) . P2 + c2 for the block's 128 rows.
+__global__ void __launch_bounds__(256) k_fno_project(const bf16 *h, const bf16 *p1t, const float *c1, const float *p2,
+                                                      float c2, float *y, int S) {
+  __shared__ float part[2][2][FZ];  // [column sb][channel half][z]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int sb = wave >> 1, wh = wave & 1, wn = wh * 64;
+  const int s = blockIdx.x * 2 + sb;
+  const bool live = s < S;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  if (live) pointwise_tiles(h + static_cast<int64_t>(s) * FZ * FC, p1t, wn, lane, acc);
+  // per-lane products with P2, summed over the lane's two channel tiles
+  float t[2][16];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t[i][r] = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int n = wn + j * 32 + (lane & 31);
+    const float cn = c1[n], pn = p2[n];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) t[i][r] += gelu_f(acc[i][j][r] + cn) * pn;
+  }
+  // sum over the 32 channels held by lanes of the same half-wave
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      float v = t[i][r];
+#pragma unroll
+      for (int o = 16; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+      t[i][r] = v;
+    }
+  if ((lane & 31) == 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) part[sb][wh][i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)] = t[i][r];
+  }
+  __syncthreads();
+  if (tid < 2 * FZ) {
+    const int b2 = tid / FZ, z = tid % FZ, ss = blockIdx.x * 2 + b2;
+    if (ss < S) y[static_cast<int64_t>(ss) * FZ + z] = part[b2][0][z] + part[b2][1][z] + c2;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // C ABI (declared in include/cupyram_hip.h)
 // ---------------------------------------------------------------------------
@@ -475,9 +708,52 @@ extern "C" int cpr_fno_fill(float *x, int64_t n, float v, void *stream) {
   return launched("cpr_fno_fill");
 }
 
+extern "C" int cpr_fno_cast_bf16_t(const float *x, void *y, int R, int Cc, void *stream) {
+  if (R <= 0 || Cc <= 0 || !x || !y) return cpr_fail("cpr_fno_cast_bf16_t: invalid arguments");
+  hipLaunchKernelGGL(k_cast_bf16_t, dim3(blocks_for(static_cast<int64_t>(R) * Cc, 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), x, static_cast<bf16 *>(y), R, Cc);
+  return launched("cpr_fno_cast_bf16_t");
+}
+
 extern "C" int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *stream) {
   if (n <= 0) return cpr_fail("cpr_fno_cast_bf16: invalid arguments");
   hipLaunchKernelGGL(k_cast_bf16, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x,
                      static_cast<bf16 *>(y), n);
   return launched("cpr_fno_cast_bf16");
+}
+
+extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const void *wt, const float *bias,
+                             const void *mh, const void *bsyn, const void *ban, void *hout, void *xh, int S,
+                             void *stream) {
+  if (S <= 0 || !hin || !wt || !hout || (syn && (!mh || !bsyn)) || (ana && (!ban || !xh)))
+    return cpr_fail("cpr_fno_layer: invalid arguments");
+  LayerArgs g;
+  g.hin = static_cast<const bf16 *>(hin);
+  g.wt = static_cast<const bf16 *>(wt);
+  g.bias = bias;
+  g.mh = static_cast<const bf16 *>(mh);
+  g.bsyn = static_cast<const bf16 *>(bsyn);
+  g.ban = static_cast<const bf16 *>(ban);
+  g.hout = static_cast<bf16 *>(hout);
+  g.xh = static_cast<bf16 *>(xh);
+  g.S = S;
+  const dim3 grid((S + 1) / 2);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!syn && !act && ana)
+    hipLaunchKernelGGL((k_fno_layer<false, false, true>), grid, dim3(256), 0, st, g);
+  else if (syn && act && ana)
+    hipLaunchKernelGGL((k_fno_layer<true, true, true>), grid, dim3(256), 0, st, g);
+  else if (syn && act && !ana)
+    hipLaunchKernelGGL((k_fno_layer<true, true, false>), grid, dim3(256), 0, st, g);
+  else
+    return cpr_fail("cpr_fno_layer: unsupported (syn, act, ana) combination");
+  return launched("cpr_fno_layer");
+}
+
+extern "C" int cpr_fno_project(const void *h, const void *p1t, const float *c1, const float *p2, float c2, float *y,
+                               int S, void *stream) {
+  if (S <= 0 || !h || !p1t || !c1 || !p2 || !y) return cpr_fail("cpr_fno_project: invalid arguments");
+  hipLaunchKernelGGL(k_fno_project, dim3((S + 1) / 2), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const bf16 *>(h), static_cast<const bf16 *>(p1t), c1, p2, c2, y, S);
+  return launched("cpr_fno_project");
 }

@@ -88,6 +88,14 @@ class FNO:
         ac, as_, sc, ss = dft_matrices(cfg.depth_points, Km)
         to = lambda a: torch.tensor(a, dtype=torch.float32, device=self.device).contiguous()  # noqa: E731
         self.ac, self.as_, self.sc, self.ss = to(ac), to(as_), to(sc), to(ss)
+        # the fused inference kernels take the transforms as bf16 hi + lo pairs:
+        # synthesis [Z][2K] (real-part rows, then imaginary), analysis [2K][Z]
+        self.fusable = (cfg.depth_points, cfg.width, cfg.modes, cfg.features) == (64, 128, 16, 128)
+        if self.fusable:
+            syn = np.concatenate([sc, ss], axis=0).T          # [Z][2K]
+            ana = np.concatenate([ac, as_], axis=0)           # [2K][Z]
+            self.bsyn = self._hi_lo(syn)
+            self.ban = self._hi_lo(ana)
         self.step_count = 0
         self._bufs = {}
         self.refresh()
@@ -105,10 +113,18 @@ class FNO:
         self.gblk = {l: torch.zeros(Km, 2 * Cw, 2 * Cw, dtype=torch.float32, device=self.device)
                      for l in range(1, self.cfg.layers + 1)}
 
+    def _hi_lo(self, a):
+        """bf16 [2][...]: hi = bf16(a), lo = bf16(a - hi), so hi + lo carries ~16 bits."""
+        t = torch.tensor(a, dtype=torch.float32)
+        hi = t.to(torch.bfloat16)
+        lo = (t - hi.float()).to(torch.bfloat16)
+        return torch.stack([hi, lo]).contiguous().to(self.device)
+
     def refresh(self):
         """Refresh the BF16 copies from the FP32 master weights."""
         for n, t in self.bf.items():
             k.cast_bf16(self.params[n], t)
+        self._fused = None  # transposed weights of the fused kernels, rebuilt on next use
         for l in range(1, self.cfg.layers + 1):
             k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes, self.cfg.width)
 
@@ -172,6 +188,46 @@ class FNO:
         pred = self._buf("pred", (R,), f32)
         k.gemm(q, self.bf["P2"], pred, R, 1, Cw, bias=self.params["c2"], epi=k.EPI_F32)
         self._h = h
+        return pred
+
+    def forward_fused(self, x):
+        """Inference with the fused kernels: per layer one mode-mixing product
+        and one fused layer kernel (pointwise product, spectral synthesis,
+        bias, GELU, next layer's analysis), then the fused projection.
+        x: bf16 [S, Z, F] -> prediction f32 [S*Z] (normalised TL)."""
+        cfg = self.cfg
+        if not self.fusable:
+            raise ValueError("the fused kernels are built for Z=64, width=128, modes=16, features=128")
+        S, Z, F = x.shape
+        if Z != cfg.depth_points or F != cfg.features or x.dtype != torch.bfloat16:
+            raise ValueError(f"input must be bf16 [S, {cfg.depth_points}, {cfg.features}]")
+        x = x.contiguous()
+        bf = torch.bfloat16
+        if self._fused is None or self._fused[0] != self.step_count:
+            # transposed ([out][in]) bf16 products and the output bias as a host float
+            wt = {}
+            for n in self.bf:
+                if n.startswith(("W0", "Wp", "P1")):
+                    w = self.params[n]
+                    wt[n] = torch.empty(w.shape[1], w.shape[0], dtype=bf, device=self.device)
+                    k.cast_bf16_t(w, wt[n])
+            self._fused = (self.step_count, wt, float(self.params["c2"].item()))
+        wt, c2 = self._fused[1], self._fused[2]
+        Cw, Km, L = cfg.width, cfg.modes, cfg.layers
+        R = S * Z
+        bf = torch.bfloat16
+        h = [self._buf("fh0", (R, Cw), bf), self._buf("fh1", (R, Cw), bf)]
+        xh = self._buf("fxh", (Km, S, 2 * Cw), bf)
+        mh = self._buf("fmh", (Km, S, 2 * Cw), bf)
+        k.fno_layer(x, wt["W0"], self.params["b0"], h[0], S, ban=self.ban, xh=xh, act=False)
+        for l in range(1, L + 1):
+            k.gemm(xh, self.blk[l], mh, S, 2 * Cw, 2 * Cw, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+                   sc=S * 2 * Cw, epi=k.EPI_BF16)
+            last = l == L
+            k.fno_layer(h[(l - 1) % 2], wt[f"Wp{l}"], self.params[f"bp{l}"], h[l % 2], S, mh=mh,
+                        bsyn=self.bsyn, ban=None if last else self.ban, xh=None if last else xh, act=True)
+        pred = self._buf("fpred", (R,), torch.float32)
+        k.fno_project(h[L % 2], wt["P1"], self.params["c1"], self.params["P2"].reshape(-1), c2, pred, S)
         return pred
 
     # -- loss and backward -----------------------------------------------
@@ -241,3 +297,11 @@ class FNO:
         for l in range(1, self.cfg.layers + 1):
             k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes,
                           self.cfg.width)
+
+
+def infer(net, x):
+    """Inference through the fused kernels when the network's shape has them,
+    else the layer-by-layer forward pass."""
+    if getattr(net, "fusable", False):
+        return net.forward_fused(x)
+    return net.forward(x, train=False)

@@ -150,3 +150,46 @@ def cast_bf16(x, y):
     _need(x, torch.float32, x.numel(), "cast x")
     _need(y, torch.bfloat16, x.numel(), "cast y")
     _abi.check(_abi.lib().cpr_fno_cast_bf16(_p(x), _p(y), x.numel(), _stream()))
+
+
+FUSED_Z, FUSED_C, FUSED_K = 64, 128, 16
+
+
+def fno_layer(hin, wt, bias, hout, S, *, mh=None, bsyn=None, ban=None, xh=None, act=True):
+    """Fused inference layer (cpr_fno_layer): hout = act(hin W + bias [+ syn(mh)]),
+    and xh = the truncated DFT of hout when ban/xh are given."""
+    Z, Cw, K2 = FUSED_Z, FUSED_C, 2 * FUSED_K
+    syn, ana = mh is not None, xh is not None
+    _need(hin, torch.bfloat16, S * Z * Cw, "fno_layer hin")
+    _need(wt, torch.bfloat16, Cw * Cw, "fno_layer wt")
+    _need(hout, torch.bfloat16, S * Z * Cw, "fno_layer hout")
+    if bias is not None:
+        _need(bias, torch.float32, Cw, "fno_layer bias")
+    if syn:
+        _need(mh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer mh")
+        _need(bsyn, torch.bfloat16, 2 * Z * K2, "fno_layer bsyn")
+    if ana:
+        _need(ban, torch.bfloat16, 2 * K2 * Z, "fno_layer ban")
+        _need(xh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer xh")
+    _abi.check(_abi.lib().cpr_fno_layer(int(syn), int(act), int(ana), _p(hin), _p(wt), _p(bias), _p(mh),
+                                        _p(bsyn), _p(ban), _p(hout), _p(xh), int(S), _stream()))
+
+
+def fno_project(h, p1t, c1, p2, c2, y, S):
+    """y = GELU(h P1 + c1) P2 + c2 (cpr_fno_project)."""
+    Z, Cw = FUSED_Z, FUSED_C
+    _need(h, torch.bfloat16, S * Z * Cw, "fno_project h")
+    _need(p1t, torch.bfloat16, Cw * Cw, "fno_project p1t")
+    _need(c1, torch.float32, Cw, "fno_project c1")
+    _need(p2, torch.float32, Cw, "fno_project p2")
+    _need(y, torch.float32, S * Z, "fno_project y")
+    _abi.check(_abi.lib().cpr_fno_project(_p(h), _p(p1t), _p(c1), _p(p2), C.c_float(float(c2)), _p(y), int(S),
+                                          _stream()))
+
+
+def cast_bf16_t(x, y):
+    """y [C][R] bf16 = x [R][C] f32 transposed."""
+    R, Cc = x.shape
+    _need(x, torch.float32, R * Cc, "cast_bf16_t x")
+    _need(y, torch.bfloat16, R * Cc, "cast_bf16_t y")
+    _abi.check(_abi.lib().cpr_fno_cast_bf16_t(_p(x), _p(y), int(R), int(Cc), _stream()))

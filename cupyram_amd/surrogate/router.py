@@ -23,6 +23,7 @@ import numpy as np
 import torch
 
 from .encode import Domain, Queries, denormalise_tl, encode
+from .fno import infer
 from .train import field_rms, pe_fields
 
 
@@ -45,7 +46,8 @@ class Router:
     def __init__(self, dom: Domain, members, threshold_db=3.0, spot_fraction=0.01, seed=0, precision="c64",
                  batch=128, log=None, pe=pe_fields, device="cuda"):
         """members: trained networks (anything with forward(x, train=False)
-        returning normalised TL [n*nr*nzp]); pe(dom, queries, precision,
+        returning normalised TL [n*nr*nzp]; networks of the fused shape are
+        run through forward_fused); pe(dom, queries, precision,
         log=...) -> (TL [n, nr, nzp] dB, seconds, cell-steps) answers the
         routed queries (the complex64 march by default)."""
         if not members:
@@ -74,7 +76,7 @@ class Router:
             s1 = torch.zeros(len(idx), dom.nr, dom.nzp, device=dev)
             s2 = torch.zeros_like(s1)
             for m in self.members:
-                y = denormalise_tl(dom, m.forward(x, train=False).view(len(idx), dom.nr, dom.nzp))
+                y = denormalise_tl(dom, infer(m, x).view(len(idx), dom.nr, dom.nzp))
                 y = y.clamp(dom.tl_lo, dom.tl_hi)
                 s1 += y
                 s2 += y * y

@@ -24,7 +24,7 @@ import torch
 from ..cupyram import CuPyRAM
 from . import dataset as D
 from .encode import ENCODING, Domain, Queries, denormalise_tl, encode, global_features, normalise_tl
-from .fno import FNO, FNOConfig
+from .fno import FNO, FNOConfig, infer
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "weights")
@@ -105,7 +105,7 @@ def predict(dom, fno, q, batch=128, cfg=FNOConfig()):
     out = torch.empty(n, dom.nr, dom.nzp, dtype=torch.float32, device="cuda")
     for a in range(0, n, batch):
         idx = np.arange(a, min(n, a + batch))
-        y = fno.forward(encode(dom, q.take(idx), cfg.features), train=False)
+        y = infer(fno, encode(dom, q.take(idx), cfg.features))
         out[a:a + len(idx)] = denormalise_tl(dom, y.view(len(idx), dom.nr, dom.nzp))
     return out
 

@@ -178,7 +178,24 @@ int cpr_fno_mse(const float *pred, int ldp, const float *target, const float *ma
 int cpr_fno_adam(float *w, const float *g, float *m1, float *m2, void *wb, int64_t n, float lr, float b1, float b2,
                  float eps, int step, float wd, void *stream);
 int cpr_fno_fill(float *x, int64_t n, float v, void *stream);
+/* Fused inference layer for Z = 64 depth points, C = F = 128 channels, 16
+ * modes; two depth columns per block.  hout [S][64][128] bf16 =
+ * act(hin . W + bias + syn(mh)), with wt = W transposed ([out][in] bf16),
+ * syn(mh)[s][z][c] = sum_k' bsyn[z][k'] mh(k', s, c) over the 32 real and
+ * imaginary mixed modes mh [16][S][256], bsyn [2][64][32] the hi and lo bf16
+ * parts of the synthesis matrix, act = GELU; and (ana) xh [16][S][256] =
+ * the truncated DFT of hout with ban [2][32][64] (hi, lo parts of the
+ * analysis matrix).  Supported (syn, act, ana): (0,0,1) lift, (1,1,1) inner
+ * layer, (1,1,0) last layer. */
+int cpr_fno_layer(int syn, int act, int ana, const void *hin, const void *wt, const float *bias, const void *mh,
+                  const void *bsyn, const void *ban, void *hout, void *xh, int S, void *stream);
+/* y [S*64] f32 = GELU(h . P1 + c1) . P2 + c2 with p1t = P1 transposed
+ * ([out][in] bf16), p2 [128] f32. */
+int cpr_fno_project(const void *h, const void *p1t, const float *c1, const float *p2, float c2, float *y, int S,
+                    void *stream);
 int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *stream);
+/* y [Cc][R] bf16 = x [R][Cc] f32 transposed. */
+int cpr_fno_cast_bf16_t(const float *x, void *y, int R, int Cc, void *stream);
 
 #ifdef __cplusplus
 }

@@ -259,3 +259,41 @@ def test_training_reduces_loss_on_a_teacher_network(require_gpu):
     torch.cuda.synchronize()
     print(f"teacher fit: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < 0.3 * losses[0]
+
+
+@pytest.mark.parametrize("S", [6, 7])
+def test_fused_inference_matches_float64_reference(require_gpu, S):
+    """The fused inference kernels (pointwise + spectral synthesis + GELU +
+    next analysis in one layer kernel, fused projection) against the float64
+    network and against the unfused path; S = 7 leaves the second column of
+    the last block empty."""
+    cfg, x, tgt, mask = _small_batch(12, S=S)
+    fno = FNO(cfg, seed=3)
+    for n in fno.names:
+        if n.startswith(("b", "c")):
+            fno.params[n].normal_(0.0, 0.1)
+    fno.refresh()
+    fused = fno.forward_fused(x.cuda()).clone()
+    unfused = fno.forward(x.cuda(), train=False).clone()
+    torch.cuda.synchronize()
+    ref = R.forward(R.reference_params(fno), x.double(), cfg)
+    e_ref, e_unf = _rel(fused.cpu(), ref), _rel(fused.cpu(), unfused.cpu())
+    print(f"fused vs float64 {e_ref:.2e}, vs unfused {e_unf:.2e}, unfused vs float64 {_rel(unfused.cpu(), ref):.2e}")
+    assert e_ref < 2e-2
+    assert e_unf < 2e-2
+
+
+def test_fused_inference_follows_training_steps(require_gpu):
+    """The fused path rebuilds its transposed weights after optimiser steps."""
+    cfg, x, tgt, mask = _small_batch(13)
+    fno = FNO(cfg, seed=4)
+    a = fno.forward_fused(x.cuda()).clone()
+    pred = fno.forward(x.cuda(), train=True)
+    _, dpred = fno.loss_grad(pred, tgt.cuda(), mask.cuda())
+    fno.backward(dpred)
+    fno.step(1e-2)
+    b = fno.forward_fused(x.cuda()).clone()
+    c = fno.forward(x.cuda(), train=False).clone()
+    torch.cuda.synchronize()
+    assert not torch.equal(a, b)
+    assert _rel(b.cpu(), c.cpu()) < 2e-2
