@@ -55,6 +55,7 @@ struct GemmArgs {
   float alpha;
   int splits;           // split-K factor (atomic epilogue when > 1)
   int kchunk;
+  int vec;              // operands allow 16-byte loads: 16-byte aligned bases, ld and batch strides % 8 == 0
 };
 
 enum { EPI_F32 = 0, EPI_BF16 = 1, EPI_GELU_BF16 = 2, EPI_ATOMIC = 3 };
@@ -63,8 +64,8 @@ enum { EPI_F32 = 0, EPI_BF16 = 1, EPI_GELU_BF16 = 2, EPI_ATOMIC = 3 };
 
 template <bool TA, bool TB, int EPI>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
-  __shared__ bf16 As[BM][LDP];
-  __shared__ bf16 Bs[BN][LDP];
+  __shared__ __attribute__((aligned(16))) bf16 As[BM][LDP];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[BN][LDP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int z = blockIdx.z;
   const int batch = z / g.splits, split = z % g.splits;
@@ -84,40 +85,74 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
   const bf16 zero = static_cast<bf16>(0.0f);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    // stage op(A)[m0..m0+127][k0..k0+31] into As[m][k]
+    // stage op(A)[m0..m0+127][k0..k0+31] into As[m][k] and op(B) into Bs[n][k];
+    // whole tiles of 16-byte-loadable operands take 16-byte global loads
+    const bool kfull = k0 + BK <= kend;
+    const bool va = g.vec && kfull && m0 + BM <= g.M, vb = g.vec && kfull && n0 + BN <= g.N;
     if (!TA) {
       const int r = tid >> 1, kk = (tid & 1) * 16;
       const int m = m0 + r;
+      if (va) {
+        const bf16 *src = A + static_cast<int64_t>(m) * g.lda + k0 + kk;
+        *reinterpret_cast<bf16x8 *>(&As[r][kk]) = *reinterpret_cast<const bf16x8 *>(src);
+        *reinterpret_cast<bf16x8 *>(&As[r][kk + 8]) = *reinterpret_cast<const bf16x8 *>(src + 8);
+      } else {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int k = k0 + kk + i;
-        As[r][kk + i] = (m < g.M && k < kend) ? A[static_cast<int64_t>(m) * g.lda + k] : zero;
+        for (int i = 0; i < 16; ++i) {
+          const int k = k0 + kk + i;
+          As[r][kk + i] = (m < g.M && k < kend) ? A[static_cast<int64_t>(m) * g.lda + k] : zero;
+        }
       }
     } else {
       const int kk = tid >> 3, mm = (tid & 7) * 16;
       const int k = k0 + kk;
+      if (va) {
+        const bf16 *src = A + static_cast<int64_t>(k) * g.lda + m0 + mm;
+        const bf16x8 v0 = *reinterpret_cast<const bf16x8 *>(src), v1 = *reinterpret_cast<const bf16x8 *>(src + 8);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int m = m0 + mm + i;
-        As[mm + i][kk] = (m < g.M && k < kend) ? A[static_cast<int64_t>(k) * g.lda + m] : zero;
+        for (int i = 0; i < 8; ++i) {
+          As[mm + i][kk] = v0[i];
+          As[mm + 8 + i][kk] = v1[i];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int m = m0 + mm + i;
+          As[mm + i][kk] = (m < g.M && k < kend) ? A[static_cast<int64_t>(k) * g.lda + m] : zero;
+        }
       }
     }
-    // stage op(B)[k0..k0+31][n0..n0+127] into Bs[n][k]
     if (!TB) {
       const int kk = tid >> 3, nn = (tid & 7) * 16;
       const int k = k0 + kk;
+      if (vb) {
+        const bf16 *src = B + static_cast<int64_t>(k) * g.ldb + n0 + nn;
+        const bf16x8 v0 = *reinterpret_cast<const bf16x8 *>(src), v1 = *reinterpret_cast<const bf16x8 *>(src + 8);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int n = n0 + nn + i;
-        Bs[nn + i][kk] = (n < g.N && k < kend) ? B[static_cast<int64_t>(k) * g.ldb + n] : zero;
+        for (int i = 0; i < 8; ++i) {
+          Bs[nn + i][kk] = v0[i];
+          Bs[nn + 8 + i][kk] = v1[i];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int n = n0 + nn + i;
+          Bs[nn + i][kk] = (n < g.N && k < kend) ? B[static_cast<int64_t>(k) * g.ldb + n] : zero;
+        }
       }
     } else {
       const int r = tid >> 1, kk = (tid & 1) * 16;
       const int n = n0 + r;
+      if (vb) {
+        const bf16 *src = B + static_cast<int64_t>(n) * g.ldb + k0 + kk;
+        *reinterpret_cast<bf16x8 *>(&Bs[r][kk]) = *reinterpret_cast<const bf16x8 *>(src);
+        *reinterpret_cast<bf16x8 *>(&Bs[r][kk + 8]) = *reinterpret_cast<const bf16x8 *>(src + 8);
+      } else {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int k = k0 + kk + i;
-        Bs[r][kk + i] = (n < g.N && k < kend) ? B[static_cast<int64_t>(n) * g.ldb + k] : zero;
+        for (int i = 0; i < 16; ++i) {
+          const int k = k0 + kk + i;
+          Bs[r][kk + i] = (n < g.N && k < kend) ? B[static_cast<int64_t>(n) * g.ldb + k] : zero;
+        }
       }
     }
     __syncthreads();
@@ -126,13 +161,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       bf16x8 af[2], bfr[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const bf16 *pa = &As[wm + i * 32 + (lane & 31)][ks + 8 * (lane >> 5)];
-        const bf16 *pb = &Bs[wn + i * 32 + (lane & 31)][ks + 8 * (lane >> 5)];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          af[i][e] = pa[e];
-          bfr[i][e] = pb[e];
-        }
+        // rows of LDP = 40 elements (80 bytes): every fragment is 16-byte aligned
+        af[i] = *reinterpret_cast<const bf16x8 *>(&As[wm + i * 32 + (lane & 31)][ks + 8 * (lane >> 5)]);
+        bfr[i] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + i * 32 + (lane & 31)][ks + 8 * (lane >> 5)]);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -619,6 +650,8 @@ extern "C" int cpr_fno_gemm(int ta, int tb, int epi, int batch, int M, int N, in
   g.alpha = alpha;
   g.splits = splits;
   g.kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
+  const auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  g.vec = al16(A) && al16(B) && lda % 8 == 0 && ldb % 8 == 0 && sA % 8 == 0 && sB % 8 == 0 && g.kchunk % 8 == 0;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (!ta && !tb) return launch_gemm_epi<false, false>(g, epi, batch, st);
   if (!ta && tb) return launch_gemm_epi<false, true>(g, epi, batch, st);

@@ -63,9 +63,15 @@ def _lr(step, total, lr0, warm=100):
     return 1e-2 * lr0 + 0.5 * (lr0 - 1e-2 * lr0) * (1 + math.cos(math.pi * (step - warm) / max(1, total - warm)))
 
 
-def train_member(dom, q, tl, cfg=FNOConfig(), epochs=4, batch=64, lr=1e-3, seed=0, log=_log):
-    """Train one network on queries q with TL fields tl ([n, nr, nzp] GPU)."""
-    fno = FNO(cfg, seed=seed)
+def train_member(dom, q, tl, cfg=FNOConfig(), epochs=4, batch=64, lr=1e-3, seed=0, log=_log, init=None):
+    """Train one network on queries q with TL fields tl ([n, nr, nzp] GPU),
+    from random weights or from the member file `init` (further training)."""
+    if init is not None:
+        fno, dom_init = load_member(init)
+        if dom_init.name != dom.name:
+            raise ValueError(f"{init} was trained on domain {dom_init.name}, not {dom.name}")
+    else:
+        fno = FNO(cfg, seed=seed)
     target, mask = normalise_tl(dom, tl)
     n = len(q)
     steps_per_epoch = n // batch
@@ -195,6 +201,8 @@ def main(argv=None):
     ap.add_argument("--n-train", type=int, default=50000)
     ap.add_argument("--n-val", type=int, default=2000)
     ap.add_argument("--members", type=int, default=4)
+    ap.add_argument("--resume", default=None,
+                    help="directory of trained members to train further (same member files as --out writes)")
     ap.add_argument("--first-member", type=int, default=0,
                     help="index of the first member trained here (members of one ensemble may be trained "
                          "in separate runs on the same data)")
@@ -223,8 +231,9 @@ def main(argv=None):
     os.makedirs(args.out, exist_ok=True)
     preds = []
     for m in range(args.first_member, args.first_member + args.members):
+        init = os.path.join(args.resume, f"{dom.name}_m{m}.safetensors") if args.resume else None
         fno, hist = train_member(dom, q_tr, tl_tr, epochs=args.epochs, batch=args.batch, lr=args.lr,
-                                 seed=args.seed + 10 + m)
+                                 seed=args.seed + 10 + m, init=init)
         save_member(fno, dom, os.path.join(args.out, f"{dom.name}_m{m}.safetensors"))
         torch.cuda.synchronize()
         t1 = time.perf_counter()
