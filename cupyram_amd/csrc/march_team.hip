@@ -23,6 +23,8 @@
 // (outpt.py:11-58) are fused in the same persistent loop over range steps.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "march_common.h"
 
 namespace {
@@ -259,21 +261,64 @@ struct CoefRow {
   cx<R> d1, d2, d3;
 };
 
+// One half (real or imaginary parts) of a row's coefficients.
+template <class R>
+struct CoefHalf {
+  R d1, d2, d3;
+};
+
 // Where a lane keeps the coefficients of its rows: VGPRs (DS = false), or a
-// global buffer laid out [ray][row k][lane] so that a wavefront reading row k
-// touches one contiguous span (DS = true, "d-stream": frees 7 VGPRs per row
-// for deep columns; the buffer of the rays in flight stays L2/MALL-resident).
+// global buffer (DS = true, "d-stream": frees 7 VGPRs per row for deep
+// columns; the buffer of the rays in flight stays L2/MALL-resident).  The
+// buffer holds the real parts [ray][row k][lane], then the imaginary parts in
+// the same layout, so that a wavefront reading row k touches one contiguous
+// span of each, and a wavefront whose rows all have real coefficients (rows
+// above the seabed, see march_team_body) reads only the real half.
 template <class R, int M, bool DS>
 struct Coefs {
   CoefRow<R> r[M];
   __device__ CoefRow<R> get(int k) const { return r[k]; }
+  __device__ CoefHalf<R> get_re(int k) const { return CoefHalf<R>{r[k].d1.re, r[k].d2.re, r[k].d3.re}; }
+  __device__ const Coefs &view() const { return *this; }
 };
 template <class R, int M>
 struct Coefs<R, M, true> {
-  CoefRow<R> *ray;  // team-uniform: row k of lane t at ray[k * T + t]
+  CoefHalf<R> *re, *im;  // team-uniform ray bases: row k of lane t at [k * T + t]
   int T, t;
-  __device__ CoefRow<R> get(int k) const { return (ray + k * T)[t]; }
-  __device__ CoefRow<R> &at(int k) const { return (ray + k * T)[t]; }
+  // Load views: the ray bases copied through an empty asm statement, so the
+  // compiler keeps them in scalar registers and forms each row's address
+  // there (scalar base + the lane's 32-bit byte offset) instead of hoisting
+  // one 64-bit vector address per row out of the step loop and spilling them.
+  typedef const char __attribute__((address_space(1))) *gbytes;  // global memory
+  typedef const CoefHalf<R> __attribute__((address_space(1))) *ghalf;
+  struct View {
+    gbytes re, im;
+    int64_t row;   // bytes per row: T * sizeof(CoefHalf)
+    unsigned off;  // this lane's byte offset: t * sizeof(CoefHalf)
+    __device__ CoefHalf<R> half(gbytes b, int k) const {
+      const ghalf p = reinterpret_cast<ghalf>(b + k * row + off);
+      return CoefHalf<R>{p->d1, p->d2, p->d3};
+    }
+    __device__ CoefRow<R> get(int k) const {
+      const CoefHalf<R> a = half(re, k), b = half(im, k);
+      CoefRow<R> c;
+      c.d1 = cxm<R>(a.d1, b.d1);
+      c.d2 = cxm<R>(a.d2, b.d2);
+      c.d3 = cxm<R>(a.d3, b.d3);
+      return c;
+    }
+    __device__ CoefHalf<R> get_re(int k) const { return half(re, k); }
+  };
+  __device__ View view() const {
+    gbytes r = (gbytes)(re), i = (gbytes)(im);  // generic to global address space
+    asm volatile("" : "+s"(r), "+s"(i));
+    return View{r, i, static_cast<int64_t>(T) * static_cast<int64_t>(sizeof(CoefHalf<R>)),
+                static_cast<unsigned>(t) * static_cast<unsigned>(sizeof(CoefHalf<R>))};
+  }
+  __device__ void put(int k, const CoefRow<R> &c) const {
+    (re + k * T)[t] = CoefHalf<R>{c.d1.re, c.d2.re, c.d3.re};
+    (im + k * T)[t] = CoefHalf<R>{c.d1.im, c.d2.im, c.d3.im};
+  }
 };
 
 // Water and bottom media of the lane's rows g0-1 .. g0+M (FP64), kept in a
@@ -376,7 +421,7 @@ __device__ void rebuild_rows(Coefs<R, M, DS> &C, const MedCache<M> &mc, int g0, 
   khi = khi > M - 1 ? M - 1 : khi;
   if constexpr (DS) {
 #pragma unroll 1
-    for (int k = klo; k <= khi; ++k) C.at(k) = coef_from_cache<R, M>(mc, k, g0, iz, cfact);
+    for (int k = klo; k <= khi; ++k) C.put(k, coef_from_cache<R, M>(mc, k, g0, iz, cfact));
   } else {
     CoefRow<R> tmp[M];
 #pragma unroll 1
@@ -394,6 +439,65 @@ __device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
   for (int q = 0; q < M; ++q)
     if (q == k) r = v[q];
   return r;
+}
+
+// Downward elimination with the spike through the lane's rows of one Pade
+// factor (q [1 4 1] + D) x = (q [1 4 1] + rho D) u.  REAL: the wavefront's
+// coefficients are real (every row's stencil lies in the water, where the
+// wavenumber is real), so the entries q + d and the product D u take real
+// operands: 9 fewer instructions per row, and streamed rows read half the
+// bytes.  Rows past nz (k >= mv) get a zero reciprocal pivot, so c', w and d'
+// vanish there and stay zero through the upward sweep and the back
+// substitution.
+template <bool REAL, class R, int M, bool DS, bool DX>
+__device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M, DX> &X, const Coefs<R, M, DS> &C, cx<R> q,
+                                      cx<R> q4, cx<R> rho, cx<R> um, cx<R> u_next, int mv) {
+  cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(0, 0), dprev = cxm<R>(0, 0);
+  using Row = std::conditional_t<REAL, CoefHalf<R>, CoefRow<R>>;
+  const auto V = C.view();
+  auto load = [&](int k) -> Row {
+    if constexpr (REAL)
+      return V.get_re(k);
+    else
+      return V.get(k);
+  };
+  Row crn = load(0);
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const Row cr = crn;
+    if (k + 1 < M) crn = load(k + 1);
+    const cx<R> uc = L.u[k];
+    const cx<R> up = (k + 1 < M) ? L.u[k + 1] : u_next;
+    cx<R> r1, r2, r3, du;
+    if constexpr (REAL) {
+      r1 = cxm<R>(q.re + cr.d1, q.im);
+      r2 = cxm<R>(q4.re + cr.d2, q4.im);
+      r3 = cxm<R>(q.re + cr.d3, q.im);
+      du = cxm<R>(cr.d1 * um.re + cr.d2 * uc.re + cr.d3 * up.re, cr.d1 * um.im + cr.d2 * uc.im + cr.d3 * up.im);
+    } else {
+      r1 = q + cr.d1;
+      r2 = q4 + cr.d2;
+      r3 = q + cr.d3;
+      du = cfma(cfma(cr.d1 * um, cr.d2, uc), cr.d3, up);
+    }
+    const cx<R> s = um + up;
+    const cx<R> ts = cxm<R>(s.re + R(4) * uc.re, s.im + R(4) * uc.im);
+    const cx<R> rhs = cfma(q * ts, rho, du);
+    const cx<R> den = k == 0 ? r2 : cfms(r2, r1, cprev);
+    R m = rcp_real<R>(den.re * den.re + den.im * den.im);
+    m = k < mv ? m : R(0);
+    const cx<R> inv = cxm<R>(den.re * m, -(den.im * m));
+    const cx<R> cc = r3 * inv;
+    // first row: the previous unknown is y_{t-1} itself (w = -1, c' = d' = 0)
+    const cx<R> wk = k == 0 ? r1 * inv : cnmul(r1 * wprev, inv);
+    const cx<R> dk = k == 0 ? rhs * inv : cfms(rhs, r1, dprev) * inv;
+    X.set(k, cc, wk);
+    L.u[k] = dk;
+    cprev = cc;
+    wprev = wk;
+    dprev = dk;
+    um = uc;
+  }
 }
 
 template <class R, int M, int TW, bool DS, bool DX>
@@ -418,7 +522,8 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   MedCache<M> mc;
   const double cfact = 0.5 / (P.dz * P.dz);
   if constexpr (DS) {
-    C.ray = static_cast<CoefRow<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T;
+    C.re = static_cast<CoefHalf<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T;
+    C.im = static_cast<CoefHalf<R> *>(a.dbuf) + (static_cast<int64_t>(a.nrays) + b) * M * T;
     C.T = T;
     C.t = t;
   }
@@ -445,6 +550,8 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
       rebuild_rows<R, M, DS>(C, mc, g0, st.iz, cfact, lo - g0, hi + 1 - g0);
     }
     fresh = false;
+    // every row stencil of this wavefront above the seabed row: real coefficients
+    const bool wreal = __builtin_amdgcn_ballot_w64(g0 + M > st.iz) == 0;
     const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
     for (int j = 0; j < P.np; ++j) {
       // Pade factor (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1],
@@ -461,41 +568,13 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
         q4 = cxm<R>(static_cast<R>(4.0 * a1d * ir), static_cast<R>(4.0 * a1d * ii));
         rho = cxm<R>(static_cast<R>(p1r * ir - p1i * ii), static_cast<R>(p1r * ii + p1i * ir));
       }
-      // 1. halo rows and downward elimination with the spike.  Rows past nz
-      // (k >= mv) get a zero reciprocal pivot, so c', w and d' vanish there
-      // and stay zero through the upward sweep and the back substitution.
+      // 1. halo rows and downward elimination with the spike
       cx<R> um, u_next;
       team.halo(L.u[M - 1], L.u[0], um, u_next);
-      cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(0, 0), dprev = cxm<R>(0, 0);
-      CoefRow<R> crn = C.get(0);
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        const CoefRow<R> cr = crn;
-        if (k + 1 < M) crn = C.get(k + 1);
-        const cx<R> uc = L.u[k];
-        const cx<R> up = (k + 1 < M) ? L.u[k + 1] : u_next;
-        const cx<R> r1 = q + cr.d1;
-        const cx<R> r2 = q4 + cr.d2;
-        const cx<R> r3 = q + cr.d3;
-        const cx<R> s = um + up;
-        const cx<R> ts = cxm<R>(s.re + R(4) * uc.re, s.im + R(4) * uc.im);
-        const cx<R> du = cfma(cfma(cr.d1 * um, cr.d2, uc), cr.d3, up);
-        const cx<R> rhs = cfma(q * ts, rho, du);
-        const cx<R> den = k == 0 ? r2 : cfms(r2, r1, cprev);
-        R m = rcp_real<R>(den.re * den.re + den.im * den.im);
-        m = k < mv ? m : R(0);
-        const cx<R> inv = cxm<R>(den.re * m, -(den.im * m));
-        const cx<R> cc = r3 * inv;
-        // first row: the previous unknown is y_{t-1} itself (w = -1, c' = d' = 0)
-        const cx<R> wk = k == 0 ? r1 * inv : cnmul(r1 * wprev, inv);
-        const cx<R> dk = k == 0 ? rhs * inv : cfms(rhs, r1, dprev) * inv;
-        X.set(k, cc, wk);
-        L.u[k] = dk;
-        cprev = cc;
-        wprev = wk;
-        dprev = dk;
-        um = uc;
-      }
+      if (wreal)
+        eliminate_down<true>(L, X, C, q, q4, rho, um, u_next, mv);
+      else
+        eliminate_down<false>(L, X, C, q, q4, rho, um, u_next, mv);
       // 2. upward sweep: x_k = D_k - W_k y_{t-1} - C_k y_t for k < M-1
 #pragma unroll
       for (int k = M - 3; k >= 0; --k) {
