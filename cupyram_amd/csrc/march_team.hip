@@ -734,7 +734,8 @@ int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
 // Rows per lane and where the lane keeps its state: complex64 holds
 // everything in VGPRs up to 12 rows, streams the Galerkin coefficients at 16,
 // 24 and 32 rows, and also the elimination arrays at 40, 48 and 64 rows;
-// complex128 likewise at up to 6, 8-12 and 16-32 rows.
+// complex128 likewise at up to 6, 8-12 and 16-40 rows (40 x 512 lanes hold
+// the deepest columns, 20,480 rows).
 bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 16 : rows >= 8; }
 bool team_rows_elim_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 40 : rows >= 16; }
 
@@ -744,14 +745,15 @@ bool team_shape_supported(int precision, int rows, int lanes) {
   if (precision == CPR_C64)
     return rows == 4 || rows == 8 || rows == 12 || rows == 16 || rows == 24 || rows == 32 || rows == 40 ||
            rows == 48 || rows == 64;
-  return rows == 2 || rows == 4 || rows == 6 || rows == 8 || rows == 12 || rows == 16 || rows == 24 || rows == 32;
+  return rows == 2 || rows == 4 || rows == 6 || rows == 8 || rows == 12 || rows == 16 || rows == 24 || rows == 32 ||
+         rows == 40;
 }
 
 void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
   const int cand64[] = {4, 8, 12, 16, 24, 32, 40, 48, 64};
-  const int cand128[] = {2, 4, 6, 8, 12, 16, 24, 32};
+  const int cand128[] = {2, 4, 6, 8, 12, 16, 24, 32, 40};
   const int *cand = precision == CPR_C64 ? cand64 : cand128;
-  const int ncand = precision == CPR_C64 ? 9 : 8;
+  const int ncand = 9;
   int best_r = 0, best_l = 0;
   double best_cost = -1;
   for (int i = 0; i < ncand; ++i) {
@@ -825,6 +827,8 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
         return launch_rows<double, 24, true, true>(a, lanes, s0, s1, st);
       case 32:
         return launch_rows<double, 32, true, true>(a, lanes, s0, s1, st);
+      case 40:
+        return launch_rows<double, 40, true, true>(a, lanes, s0, s1, st);
     }
   }
   return -1;

@@ -140,11 +140,12 @@ def test_multi_frequency_matches_single(require_gpu):
                                              ("c128", (2, 192)), ("c128", (4, 128)), ("c128", (6, 64)),
                                              ("c128", (8, 64)), ("c128", (12, 64)), ("c128", (4, 192)),
                                              ("c64", (40, 64)), ("c64", (48, 64)), ("c64", (64, 64)),
-                                             ("c128", (16, 64)), ("c128", (24, 64)), ("c128", (32, 64))])
+                                             ("c128", (16, 64)), ("c128", (24, 64)), ("c128", (32, 64)),
+                                             ("c128", (40, 64))])
 def test_every_team_shape_matches_oracle(require_gpu, precision, shape):
     """Register rows (c64 4..12, c128 2..6), streamed-coefficient rows
     (c64 16..32, c128 8..12) and streamed elimination arrays (c64 40..64,
-    c128 16..32), wave-shuffle (64 lanes) and LDS (>64 lanes) teams, on the
+    c128 16..40), wave-shuffle (64 lanes) and LDS (>64 lanes) teams, on the
     wedge (sloping bathymetry moves the water/bottom split)."""
     tol = 0.1 if precision == "c128" else 0.5
     worst = _compare_with_oracle(envs.wedge(), precision, "team", tol, grids=True, team_shape=shape)
