@@ -405,8 +405,11 @@ class CuPyRAM:
         groups = {}
         for c, sh in enumerate(shapes):
             groups.setdefault(sh, []).append(c)
+        # deepest buckets first: their rays have the longest dependent chain
+        # per step (fewest rays, largest columns), so they bound the job's end
+        # unless they start at once; the wide shallow buckets fill around them
         out = []
-        for sh, ids in sorted(groups.items(), key=lambda kv: -len(kv[1])):
+        for sh, ids in sorted(groups.items(), key=lambda kv: (-int(nz[kv[1]].max()), -len(kv[1]))):
             if sh == (0, 0):
                 out.append((numpy.array(ids), None, "thread"))
             else:
