@@ -234,3 +234,13 @@ def test_cells_uneven_batch(require_gpu):
     kw.update(ndr=2, ndz=3, zmplt=90.0)
     worst = _cells_vs_oracle(kw, "c64", "auto")
     assert worst <= 0.05, worst
+
+
+def test_benchmark_cli_batch_run(require_gpu):
+    """The batch-scaling CLI's measurement of one small batch (two frequencies)."""
+    from cupyram_amd import benchmark as B
+    bpr = B.bytes_per_ray([100.0, 150.0], 2000.0, 10.0, 10.0, 4, "c64", probe=32)
+    assert bpr > 0
+    r = B.run_batch(64, [100.0, 150.0], 2000.0, 10.0, 10.0, 4, "c64")
+    assert r["total_fields"] == 128
+    assert 0 < r["time_propagation"] < r["time_total"]

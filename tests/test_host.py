@@ -147,3 +147,22 @@ def test_shared_object_roundtrip():
     finally:
         AsyncBatcher.cleanup_shared_object(name)
     AsyncBatcher.cleanup_shared_object(name)  # second cleanup is a no-op
+
+
+def test_benchmark_cli_host_logic():
+    """Batch series and synthetic environment of the batch-scaling CLI
+    (cupyram/benchmark.py:86-194 semantics)."""
+    from cupyram_amd import benchmark as B
+    env = B.synthetic_environment(2000.0)
+    assert np.array_equal(env["rp_ss"], [0.0, 1000.0, 2000.0])
+    assert env["cw"].shape == (6, 3) and np.all(env["cw"] == 1500.0)
+    assert np.all(env["rbzb"][:, 1] == 3000.0)
+    # plenty of memory: capped at 50,000 rays, 40 % .. 120 %, multiples of 128, no repeats
+    s = B.batch_sizes(1e6, 10**12, 1, 9)
+    assert s[0] == round(20000 / 128) * 128 and s[-1] == round(60000 / 128) * 128
+    assert all(b % 128 == 0 for b in s) and len(s) == len(set(s)) and s == sorted(s)
+    # little memory: a linear series from 32
+    s = B.batch_sizes(1e6, 300 * 10**6, 1, 9)
+    assert s[0] == 32 and s[-1] <= 270
+    # frequencies share the memory
+    assert B.batch_sizes(1e6, 10**9, 2, 9)[-1] < B.batch_sizes(1e6, 10**9, 1, 9)[-1]
