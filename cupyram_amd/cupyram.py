@@ -153,47 +153,99 @@ class CuPyRAM:
         self.check_inputs(z_ss, rp_ss, cw, z_sb, rp_sb, cb, rhob, attn, rbzb)
         self.get_params(**kwargs)
 
+    @staticmethod
+    def _valid_len(A):
+        """Per-ray valid length of NaN-padded rows [B, n]: the _get_valid_slice
+        rule for 1-D arrays (up to the first NaN; all of it if the first
+        element is NaN)."""
+        nan = numpy.isnan(A)
+        first = nan.argmax(axis=1)
+        return numpy.where(nan.any(axis=1) & (first > 0), first, A.shape[1])
+
+    @staticmethod
+    def _valid_shape(A):
+        """Per-ray (rows, cols) of NaN-padded blocks [B, r, c]: the
+        _get_valid_slice rule for 2-D arrays (counts of rows and columns that
+        are not all NaN; the valid block is the leading one)."""
+        nan = numpy.isnan(A)
+        return (~nan.all(axis=2)).sum(axis=1), (~nan.all(axis=1)).sum(axis=1)
+
     def check_inputs(self, z_ss, rp_ss, cw, z_sb, rp_sb, cb, rhob, attn, rbzb):
         """Validate every ray and keep its valid (un-padded) tables
-        (cupyram.py:382-458)."""
+        (cupyram.py:382-458).  Works on the whole NaN-padded batch at once;
+        the first failing ray raises the reference's message."""
         self._status_ok = True
-        self._env = []
-        for b in range(self._batch_size):
-            zss, _ = self._get_valid_slice(numpy.asarray(z_ss[b], numpy.float64))
-            rss, _ = self._get_valid_slice(numpy.asarray(rp_ss[b], numpy.float64).ravel())
-            cwb = numpy.asarray(cw[b], numpy.float64)
-            if cwb.ndim == 1:
-                cwb = cwb[:, None]
-            cwb, _ = self._get_valid_slice(cwb)
-            zsb, _ = self._get_valid_slice(numpy.asarray(z_sb[b], numpy.float64).ravel())
-            rsb, _ = self._get_valid_slice(numpy.asarray(rp_sb[b], numpy.float64).ravel())
-            profs = []
-            for arr in (cb, rhob, attn):
-                a = numpy.asarray(arr[b], numpy.float64)
-                if a.ndim == 1:
-                    a = a[:, None]
-                profs.append(self._get_valid_slice(a)[0])
-            bt, _ = self._get_valid_slice(numpy.asarray(rbzb[b], numpy.float64).reshape(-1, 2))
-            if not (zss[0] <= self._zs_e[b] <= zss[-1]):
-                raise ValueError(f"Ray {b}: Source depth {self._zs_e[b]}m outside range [{zss[0]}, {zss[-1]}]m")
-            if not (zss[0] <= self._zr <= zss[-1]):
-                raise ValueError(f"Ray {b}: Receiver depth {self._zr}m outside range [{zss[0]}, {zss[-1]}]m")
-            if cwb.shape != (zss.size, rss.size):
-                raise ValueError(f"Ray {b}: z_ss ({zss.size}), rp_ss ({rss.size}), cw {cwb.shape} inconsistent")
-            for prof, name in zip(profs, ("cb", "rhob", "attn")):
-                if prof.shape != (zsb.size, rsb.size):
-                    raise ValueError(
-                        f"Ray {b}: z_sb ({zsb.size}), rp_sb ({rsb.size}), {name} {prof.shape} inconsistent")
-            if bt[:, 1].max() > zss[-1]:
-                raise ValueError(f"Ray {b}: Max bathy ({bt[:, 1].max()}m) > max SSP depth ({zss[-1]}m)")
-            if numpy.any(numpy.diff(zss) < 0) or numpy.any(numpy.diff(zsb) < 0):
-                raise ValueError(f"Ray {b}: profile depths must be non-decreasing")
-            self._env.append({"z_ss": zss, "rp_ss": rss, "cw": cwb, "z_sb": zsb, "rp_sb": rsb,
-                              "cb": profs[0], "rhob": profs[1], "attn": profs[2], "rbzb": bt})
+        B = self._batch_size
+        f64 = lambda a: numpy.asarray(a, numpy.float64)  # noqa: E731
+        T = {"z_ss": f64(z_ss).reshape(B, -1), "rp_ss": f64(rp_ss).reshape(B, -1),
+             "z_sb": f64(z_sb).reshape(B, -1), "rp_sb": f64(rp_sb).reshape(B, -1),
+             "rbzb": f64(rbzb).reshape(B, -1, 2)}
+        for name, arr in (("cw", cw), ("cb", cb), ("rhob", rhob), ("attn", attn)):
+            a = f64(arr)
+            T[name] = a.reshape(B, a.shape[1], -1) if a.ndim >= 2 else a.reshape(B, -1, 1)
+        L = {k: self._valid_len(T[k]) for k in ("z_ss", "rp_ss", "z_sb", "rp_sb")}
+        S = {k: self._valid_shape(T[k]) for k in ("cw", "cb", "rhob", "attn", "rbzb")}
+        ar = numpy.arange(B)
+        zfirst = T["z_ss"][:, 0]
+        zlast = T["z_ss"][ar, L["z_ss"] - 1]
+        bt_rows = S["rbzb"][0]
+        bt_z = numpy.where(numpy.arange(T["rbzb"].shape[1])[None, :] < bt_rows[:, None], T["rbzb"][:, :, 1], -numpy.inf)
+        bt_max = bt_z.max(axis=1)
+
+        def decreasing(A, n):
+            d = numpy.diff(A, axis=1)
+            ok = numpy.arange(d.shape[1])[None, :] < (n - 1)[:, None]
+            return numpy.any(ok & (d < 0), axis=1)
+
+        checks = [
+            (~((zfirst <= self._zs_e) & (self._zs_e <= zlast)),
+             lambda b: f"Ray {b}: Source depth {self._zs_e[b]}m outside range [{zfirst[b]}, {zlast[b]}]m"),
+            (~((zfirst <= self._zr) & (self._zr <= zlast)),
+             lambda b: f"Ray {b}: Receiver depth {self._zr}m outside range [{zfirst[b]}, {zlast[b]}]m"),
+            ((S["cw"][0] != L["z_ss"]) | (S["cw"][1] != L["rp_ss"]),
+             lambda b: f"Ray {b}: z_ss ({L['z_ss'][b]}), rp_ss ({L['rp_ss'][b]}), "
+                       f"cw ({S['cw'][0][b]}, {S['cw'][1][b]}) inconsistent"),
+        ]
+        for name in ("cb", "rhob", "attn"):
+            checks.append(((S[name][0] != L["z_sb"]) | (S[name][1] != L["rp_sb"]),
+                           lambda b, name=name: f"Ray {b}: z_sb ({L['z_sb'][b]}), rp_sb ({L['rp_sb'][b]}), "
+                                                f"{name} ({S[name][0][b]}, {S[name][1][b]}) inconsistent"))
+        checks.append((bt_max > zlast, lambda b: f"Ray {b}: Max bathy ({bt_max[b]}m) > max SSP depth ({zlast[b]}m)"))
+        checks.append((decreasing(T["z_ss"], L["z_ss"]) | decreasing(T["z_sb"], L["z_sb"]),
+                       lambda b: f"Ray {b}: profile depths must be non-decreasing"))
+        bad = numpy.zeros(B, bool)
+        for m, _ in checks:
+            bad |= m
+        if bad.any():
+            b = int(numpy.argmax(bad))
+            for m, msg in checks:
+                if m[b]:
+                    raise ValueError(msg(b))
+        self._tables, self._lens, self._shapes = T, L, S
+        self._zss_last = zlast
+        self._env = [{"z_ss": T["z_ss"][b, :L["z_ss"][b]], "rp_ss": T["rp_ss"][b, :L["rp_ss"][b]],
+                      "cw": T["cw"][b, :S["cw"][0][b], :S["cw"][1][b]],
+                      "z_sb": T["z_sb"][b, :L["z_sb"][b]], "rp_sb": T["rp_sb"][b, :L["rp_sb"][b]],
+                      "cb": T["cb"][b, :S["cb"][0][b], :S["cb"][1][b]],
+                      "rhob": T["rhob"][b, :S["rhob"][0][b], :S["rhob"][1][b]],
+                      "attn": T["attn"][b, :S["attn"][0][b], :S["attn"][1][b]],
+                      "rbzb": T["rbzb"][b, :bt_rows[b], :S["rbzb"][1][b]]} for b in range(B)]
         e0 = self._env[0]
         self.rd_ss = e0["rp_ss"].size > 1
         self.rd_sb = e0["rp_sb"].size > 1
         self.rd_bt = e0["rbzb"].shape[0] > 1
+
+    def _masked(self, name, col=None):
+        """Valid entries of table `name` for every ray, NaN elsewhere (or the
+        given column of the blocks)."""
+        A = self._tables[name]
+        if name in self._lens:
+            return numpy.where(numpy.arange(A.shape[1])[None, :] < self._lens[name][:, None], A, numpy.nan)
+        rows, cols = self._shapes[name]
+        ok = (numpy.arange(A.shape[1])[None, :, None] < rows[:, None, None]) & \
+             (numpy.arange(A.shape[2])[None, None, :] < cols[:, None, None])
+        M = numpy.where(ok, A, numpy.nan)
+        return M if col is None else M[:, :, col]
 
     def get_params(self, **kwargs):
         """Defaults of cupyram.py:460-526; per-ray arrays are an extension."""
@@ -207,7 +259,8 @@ class CuPyRAM:
             self._c0 = kwargs["c0"]
             self._c0_array = numpy.full(self._batch_size, float(kwargs["c0"]))
         else:
-            self._c0_array = numpy.array([float(numpy.nanmean(e["cw"][:, 0])) for e in self._env])
+            with numpy.errstate(invalid="ignore"):
+                self._c0_array = numpy.nanmean(self._masked("cw", 0), axis=1)
             self._c0 = float(numpy.mean(self._c0_array))
         max_freq = float(numpy.max(self._freqs))
         self._lambda = self._c0 / max_freq
@@ -222,9 +275,10 @@ class CuPyRAM:
         self._ndr = int(self._ndr_e[0]) if numpy.ndim(kwargs.get("ndr", 1)) == 0 else self._ndr_e
         self._ndz = int(self._ndz_e[0]) if numpy.ndim(kwargs.get("ndz", 1)) == 0 else self._ndz_e
         self._zmplt_given = "zmplt" in kwargs
-        self._zmplt = kwargs.get("zmplt", max(float(numpy.max(e["rbzb"][:, 1])) for e in self._env))
-        rmax_default = max(max(float(numpy.max(e["rp_ss"])), float(numpy.max(e["rp_sb"])),
-                               float(numpy.max(e["rbzb"][:, 0]))) for e in self._env)
+        self._bt_zmax = numpy.nanmax(self._masked("rbzb", 1), axis=1)
+        self._zmplt = kwargs["zmplt"] if "zmplt" in kwargs else float(numpy.max(self._bt_zmax))
+        rmax_default = max(float(numpy.nanmax(self._masked("rp_ss"))), float(numpy.nanmax(self._masked("rp_sb"))),
+                           float(numpy.nanmax(self._masked("rbzb", 0))))
         self._rmax = kwargs.get("rmax", rmax_default)
         self._rmax_input = self._rmax
         self._ns = int(kwargs.get("ns", CuPyRAM._ns_default))
@@ -259,15 +313,15 @@ class CuPyRAM:
         else:
             lam_e = numpy.full(ne, self._lambda)
         # seabed depths are relative to the last water-profile depth (cupyram.py:580-587)
-        zsb_abs = [e["z_sb"] + e["z_ss"][-1] for e in self._env]
-        zmax_sb = numpy.array([float(numpy.nanmax(z)) for z in zsb_abs])
+        zsb_abs = self._masked("z_sb") + self._zss_last[:, None]
+        zmax_sb = numpy.nanmax(zsb_abs, axis=1)
         if self._per_ray_grid:
             zmax_e = zmax_sb + self._lyrw * lam_e
         else:
             zmax_e = numpy.full(ne, float(numpy.max(zmax_sb)) + self._lyrw * self._lambda)
         nz_e = numpy.floor(zmax_e / dz_e).astype(numpy.int64) - 1
         if self._per_ray_grid and not self._zmplt_given:
-            zmplt_e = numpy.array([float(numpy.max(e["rbzb"][:, 1])) for e in self._env])
+            zmplt_e = self._bt_zmax.copy()
         else:
             zmplt_e = self._per_env(self._zmplt, ne)
         nzplt_e = numpy.floor(zmplt_e / dz_e).astype(numpy.int64)
@@ -281,39 +335,50 @@ class CuPyRAM:
             raise ValueError("depth grid too coarse (nz < 3)")
         if numpy.any(nvz_e * ndz_e > nz_e + 1):
             raise ValueError("zmplt lies below the computational grid")
-        # environment tables, extended bathymetry (cupyram.py:536-568)
+        # environment tables, extended bathymetry (cupyram.py:536-568), packed
+        # table by table: every ray's valid block of one table, then the next
+        T, Ls, Sh = self._tables, self._lens, self._shapes
+        ar = numpy.arange(ne)
+        bt_rows = Sh["rbzb"][0]
+        last = bt_rows - 1
+        ext = T["rbzb"][:, :, :2]
+        ext = numpy.concatenate([ext, numpy.full((ne, 2, 2), numpy.nan)], axis=1)
+        grow = T["rbzb"][ar, last, 0] < rmax_e          # extend to rmax with the last depth
+        ext[ar[grow], bt_rows[grow], 0] = rmax_e[grow]
+        ext[ar[grow], bt_rows[grow], 1] = T["rbzb"][ar[grow], last[grow], 1]
+        n_bt = bt_rows + grow
+        one = n_bt == 1                                 # a single point: a flat second one 1 m on
+        ext[ar[one], 1, 0] = ext[ar[one], 0, 0] + 1.0
+        ext[ar[one], 1, 1] = ext[ar[one], 0, 1]
+        n_bt = numpy.where(one, 2, n_bt)
+        iz0_e = numpy.clip(numpy.floor(ext[:, 0, 1] / dz_e).astype(numpy.int64), 1, nz_e - 1)
         chunks, envs, off = [], numpy.zeros(ne, _abi.ENV_DTYPE), 0
-        iz0_e = numpy.zeros(ne, numpy.int64)
 
-        def put(a):
+        def pack(field, A, rows, cols=None):
+            """Append every ray's leading rows (x cols) block of A; set offsets."""
             nonlocal off
-            a = numpy.ascontiguousarray(a, numpy.float64).ravel()
-            chunks.append(a)
-            o = off
-            off += a.size
-            return o
+            if cols is None:
+                ok = numpy.arange(A.shape[1])[None, :] < rows[:, None]
+                size = rows
+            else:
+                ok = (numpy.arange(A.shape[1])[None, :, None] < rows[:, None, None]) & \
+                     (numpy.arange(A.shape[2])[None, None, :] < cols[:, None, None])
+                size = rows * cols
+            chunks.append(numpy.ascontiguousarray(A[ok], numpy.float64))
+            envs[field] = off + numpy.concatenate([[0], numpy.cumsum(size)[:-1]])
+            off += int(numpy.sum(size))
 
-        for b, e in enumerate(self._env):
-            bt = e["rbzb"]
-            if bt[-1, 0] < rmax_e[b]:
-                bt = numpy.vstack([bt, [[rmax_e[b], bt[-1, 1]]]])
-            if bt.shape[0] == 1:
-                bt = numpy.vstack([bt, bt])
-                bt[1, 0] = bt[0, 0] + 1.0
-            iz0_e[b] = min(max(int(numpy.floor(bt[0, 1] / dz_e[b])), 1), nz_e[b] - 1)
-            rec = envs[b]
-            rec["n_ss"], rec["n_rp_ss"] = e["cw"].shape
-            rec["n_sb"], rec["n_rp_sb"] = e["cb"].shape
-            rec["n_bt"] = bt.shape[0]
-            rec["z_ss"] = put(e["z_ss"])
-            rec["rp_ss"] = put(e["rp_ss"])
-            rec["cw"] = put(e["cw"])
-            rec["z_sb"] = put(zsb_abs[b])
-            rec["rp_sb"] = put(e["rp_sb"])
-            rec["cb"] = put(e["cb"])
-            rec["rhob"] = put(e["rhob"])
-            rec["attn"] = put(e["attn"])
-            rec["rbzb"] = put(bt)
+        envs["n_ss"], envs["n_rp_ss"] = Sh["cw"]
+        envs["n_sb"], envs["n_rp_sb"] = Sh["cb"]
+        envs["n_bt"] = n_bt
+        pack("z_ss", T["z_ss"], Ls["z_ss"])
+        pack("rp_ss", T["rp_ss"], Ls["rp_ss"])
+        pack("cw", T["cw"], *Sh["cw"])
+        pack("z_sb", zsb_abs, Ls["z_sb"])
+        pack("rp_sb", T["rp_sb"], Ls["rp_sb"])
+        for name in ("cb", "rhob", "attn"):
+            pack(name, T[name], *Sh[name])
+        pack("rbzb", ext, n_bt, numpy.full(ne, 2))
         self._env_data = numpy.concatenate(chunks) if chunks else numpy.zeros(1)
         env_of = numpy.arange(nc) // nf
         if self._freq_per_ray:

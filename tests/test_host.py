@@ -166,3 +166,39 @@ def test_benchmark_cli_host_logic():
     assert s[0] == 32 and s[-1] <= 270
     # frequencies share the memory
     assert B.batch_sizes(1e6, 10**9, 2, 9)[-1] < B.batch_sizes(1e6, 10**9, 1, 9)[-1]
+
+
+@pytest.mark.parametrize("case", ["varying_ranges", "wedge_queries"])
+def test_packed_tables_equal_per_ray_valid_slices(case):
+    """The batch-vectorised validation and packing keep, for every ray, exactly
+    the tables the per-ray rule (_get_valid_slice, the bathymetry extension to
+    rmax) gives, at the offsets the kernels read."""
+    if case == "varying_ranges":
+        kw = envs.varying_ranges()
+    else:
+        from cupyram_amd.surrogate import dataset as D
+        kw = D.march_kwargs(D.WEDGE, D.wedge_queries(7, 3))
+    m = CuPyRAM(**kw)
+    m.setup()
+    V = CuPyRAM._get_valid_slice
+    d = m._env_data
+    for b in range(m._n_env):
+        rec = m._envs[b * m._n_freq]
+        pick = lambda key, n: d[rec[key]:rec[key] + n]  # noqa: E731
+        zss = V(np.asarray(kw["z_ss"][b] if isinstance(kw["z_ss"], list) else kw["z_ss"], float))[0]
+        assert np.array_equal(pick("z_ss", rec["n_ss"]), zss)
+        cw = np.asarray(kw["cw"][b] if isinstance(kw["cw"], list) else kw["cw"], float)
+        cw = V(cw if cw.ndim == 2 else cw[:, None])[0]
+        assert np.array_equal(pick("cw", cw.size), cw.ravel())
+        zsb = V(np.asarray(kw["z_sb"][b] if isinstance(kw["z_sb"], list) else kw["z_sb"], float).ravel())[0]
+        assert np.array_equal(pick("z_sb", rec["n_sb"]), zsb + zss[-1])
+        for name in ("cb", "rhob", "attn"):
+            a = np.asarray(kw[name][b] if isinstance(kw[name], list) else kw[name], float)
+            a = V(a if a.ndim == 2 else a[:, None])[0]
+            assert np.array_equal(pick(name, a.size), a.ravel())
+        bt = V(np.asarray(kw["rbzb"][b] if isinstance(kw["rbzb"], list) else kw["rbzb"], float).reshape(-1, 2))[0]
+        rmax = float(np.ravel(kw["rmax"])[b]) if np.ndim(kw["rmax"]) else float(kw["rmax"])
+        if bt[-1, 0] < rmax:
+            bt = np.vstack([bt, [rmax, bt[-1, 1]]])
+        assert rec["n_bt"] == len(bt)
+        assert np.array_equal(pick("rbzb", 2 * len(bt)), bt.ravel())
