@@ -732,28 +732,36 @@ int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
 }  // namespace
 
 // Rows per lane and where the lane keeps its state: complex64 holds
-// everything in VGPRs up to 12 rows, streams the Galerkin coefficients at 16,
-// 24 and 32 rows, and also the elimination arrays at 40, 48 and 64 rows;
+// everything in VGPRs up to 12 rows, streams the Galerkin coefficients at 16
+// to 32 rows, and also the elimination arrays at 40, 48 and 64 rows;
 // complex128 likewise at up to 6, 8-12 and 16-40 rows (40 x 512 lanes hold
 // the deepest columns, 20,480 rows).
 bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 16 : rows >= 8; }
 bool team_rows_elim_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 40 : rows >= 16; }
 
+// Rows per lane built for each precision.  Between the powers of two, 20, 21
+// and 28 (complex64) and 10 (complex128) let a column fill its lanes with few
+// padding rows (nz = 5303: 21 x 256 = 5376 rows instead of 24 x 256 = 6144).
+constexpr int kRows64[] = {4, 8, 12, 16, 20, 21, 24, 28, 32, 40, 48, 64};
+constexpr int kRows128[] = {2, 4, 6, 8, 10, 12, 16, 24, 32, 40};
+
 bool team_shape_supported(int precision, int rows, int lanes) {
   const bool lanes_ok = lanes >= 64 && lanes <= 512 && lanes % 64 == 0;
   if (!lanes_ok) return false;
-  if (precision == CPR_C64)
-    return rows == 4 || rows == 8 || rows == 12 || rows == 16 || rows == 24 || rows == 32 || rows == 40 ||
-           rows == 48 || rows == 64;
-  return rows == 2 || rows == 4 || rows == 6 || rows == 8 || rows == 12 || rows == 16 || rows == 24 || rows == 32 ||
-         rows == 40;
+  if (precision == CPR_C64) {
+    for (int r : kRows64)
+      if (r == rows) return true;
+    return false;
+  }
+  for (int r : kRows128)
+    if (r == rows) return true;
+  return false;
 }
 
 void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
-  const int cand64[] = {4, 8, 12, 16, 24, 32, 40, 48, 64};
-  const int cand128[] = {2, 4, 6, 8, 12, 16, 24, 32, 40};
-  const int *cand = precision == CPR_C64 ? cand64 : cand128;
-  const int ncand = 9;
+  const int *cand = precision == CPR_C64 ? kRows64 : kRows128;
+  const int ncand = precision == CPR_C64 ? static_cast<int>(sizeof(kRows64) / sizeof(int))
+                                         : static_cast<int>(sizeof(kRows128) / sizeof(int));
   int best_r = 0, best_l = 0;
   double best_cost = -1;
   for (int i = 0; i < ncand; ++i) {
@@ -798,8 +806,14 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
         return launch_rows<float, 12, false>(a, lanes, s0, s1, st);
       case 16:
         return launch_rows<float, 16, true>(a, lanes, s0, s1, st);
+      case 20:
+        return launch_rows<float, 20, true>(a, lanes, s0, s1, st);
+      case 21:
+        return launch_rows<float, 21, true>(a, lanes, s0, s1, st);
       case 24:
         return launch_rows<float, 24, true>(a, lanes, s0, s1, st);
+      case 28:
+        return launch_rows<float, 28, true>(a, lanes, s0, s1, st);
       case 32:
         return launch_rows<float, 32, true>(a, lanes, s0, s1, st);
       case 40:
@@ -819,6 +833,8 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
         return launch_rows<double, 6, false>(a, lanes, s0, s1, st);
       case 8:
         return launch_rows<double, 8, true>(a, lanes, s0, s1, st);
+      case 10:
+        return launch_rows<double, 10, true>(a, lanes, s0, s1, st);
       case 12:
         return launch_rows<double, 12, true>(a, lanes, s0, s1, st);
       case 16:
