@@ -292,7 +292,11 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     if (db > 0 && dalloc(p, &p->dbuf_d, static_cast<size_t>(db))) return bail(-1);
     const int64_t xb = team_xbuf_bytes(cfg->precision, p->team_rows, p->team_lanes, cfg->nrays);
     if (xb > 0 && dalloc(p, &p->xbuf_d, static_cast<size_t>(xb))) return bail(-1);
-    if (hipMemcpy(p->team_rays_d, ids.data(), sizeof(int32_t) * cfg->nrays, hipMemcpyHostToDevice) != hipSuccess)
+    // on the plan's stream, behind dalloc's clearing of the same buffer: the
+    // stream is non-blocking, so a null-stream copy could land before it
+    if (hipMemcpyAsync(p->team_rays_d, ids.data(), sizeof(int32_t) * cfg->nrays, hipMemcpyHostToDevice,
+                       p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
       return bail(fail("upload failed"));
   }
   const size_t nline = static_cast<size_t>(std::max(cfg->nvr_max, 1)) * p->ld;
@@ -457,7 +461,8 @@ int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double 
   std::vector<double> buf;
   auto fetch = [&](const double *src, size_t count) -> int {
     buf.resize(count);
-    HIPCHK(hipMemcpy(buf.data(), src, count * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(buf.data(), src, count * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
   };
   if (tll) {
@@ -504,7 +509,8 @@ int cpr_read_cells(cpr_plan *p, double *tlc) {
   HIPCHK(hipStreamSynchronize(p->stream));
   const int64_t ld = p->ld, nvr = p->cfg.nvr_max, nvz = p->cfg.nvz_max, n = p->nrays;
   std::vector<double> buf(static_cast<size_t>(nvr * nvz * ld));
-  HIPCHK(hipMemcpy(buf.data(), p->tlcell_d, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(buf.data(), p->tlcell_d, buf.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
   for (int64_t b = 0; b < n; ++b)
     for (int64_t i = 0; i < nvz; ++i)
       for (int64_t t = 0; t < nvr; ++t) tlc[(b * nvz + i) * nvr + t] = buf[(t * nvz + i) * ld + b];
@@ -519,7 +525,8 @@ int cpr_read_field(cpr_plan *p, double *u, int nz_max) {
   const bool c64 = p->cfg.precision == CPR_C64;
   const size_t bytes = static_cast<size_t>(rows * ld) * (c64 ? 8 : 16);
   std::vector<char> buf(bytes);
-  HIPCHK(hipMemcpy(buf.data(), p->col[4], bytes, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(buf.data(), p->col[4], bytes, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
   std::memset(u, 0, sizeof(double) * 2 * n * (nz_max + 2));
   for (int64_t b = 0; b < n; ++b)
     for (int64_t i = 0; i < rows; ++i) {

@@ -245,3 +245,24 @@ def test_benchmark_cli_batch_run(require_gpu):
     r = B.run_batch(64, [100.0, 150.0], 2000.0, 10.0, 10.0, 4, "c64")
     assert r["total_fields"] == 128
     assert 0 < r["time_propagation"] < r["time_total"]
+
+
+def test_many_buckets_every_ray_marched(require_gpu):
+    """20,000 sloping-wedge rays (six team buckets, 350 to 2,234 rows, each
+    plan on its own non-blocking stream): every ray's cell-mean TL agrees
+    with the complex128 march.  Regression test for the team ray list being
+    cleared after its upload (a copy not ordered on the plan's stream), which
+    left one bucket's rays at their starting field."""
+    from cupyram_amd.surrogate import dataset as D
+    dom = D.WEDGE
+    q = D.wedge_queries(20000, 5)
+    tl = {}
+    for prec in ("c64", "c128"):
+        m = CuPyRAM(**D.march_kwargs(dom, q, precision=prec))
+        if prec == "c64":
+            m.setup()
+            assert len(m._buckets()) >= 5
+        tl[prec] = np.clip(np.asarray(m.run()["TL Cells"]), 20.0, 140.0)
+    err = np.sqrt(np.nanmean((tl["c64"] - tl["c128"]) ** 2, axis=(1, 2)))
+    assert np.all(np.isfinite(err))
+    assert err.max() <= 0.5, (int(np.argmax(err)), float(err.max()))

@@ -31,6 +31,18 @@ def test_plan_create_without_device_fails_loudly():
         m.run()
 
 
+def test_plan_copies_are_ordered_on_the_plan_stream():
+    """Plans run on non-blocking streams, which the null stream does not wait
+    for: a synchronous hipMemcpy/hipMemset beside the plan's own
+    hipMemsetAsync clearing of a buffer can land before or after it.  (An
+    upload of the team ray list that lost this race had every team march ray
+    0, leaving the other rays at their starting field.)"""
+    src = open(os.path.join(ROOT, "cupyram_amd", "csrc", "cpr_abi.hip")).read()
+    assert not re.findall(r"\bhipMemcpy\s*\(", src)
+    assert not re.findall(r"\bhipMemset\s*\(", src)
+    assert "hipMemcpyAsync(p->team_rays_d" in src
+
+
 @pytest.mark.parametrize("np_pade", [4, 6, 8])
 @pytest.mark.parametrize("ns,ip", [(1, 1), (0, 1), (1, 2)])
 @pytest.mark.parametrize("freq", [25.0, 50.0, 100.0, 200.0])
