@@ -69,11 +69,18 @@ def time_march(model, steps, warmup, launch_steps):
         wall = time.perf_counter() - t0
         ms1, n1 = plan.march_time()
         variant = plan.describe(0)
+        tll = plan.read_outputs()[0]
     finally:
         plan.close()
+    # every ray of the timed plan has stored the same number of outputs (an
+    # output never written holds p = 0, so TL >= 400 dB; a team that marched
+    # another ray's column would leave its own outputs unwritten)
+    written = np.sum(np.isfinite(tll) & (tll < 399.0), axis=1)
+    if written.min() != written.max() or written.max() == 0:
+        raise RuntimeError(f"timed march left rays unmarched: outputs per ray {written.min()}..{written.max()}")
     cells = float(np.sum(model._rays["nz"].astype(np.float64)))
     return {"wall_s": wall, "kernel_ms": ms1 - ms0, "launches": n1 - n0, "cells": cells,
-            "mcs": cells * steps / wall / 1e6, "variant": variant}
+            "mcs": cells * steps / wall / 1e6, "variant": variant, "outputs_per_ray": int(written.max())}
 
 
 def cpu_baseline(kw, sample_rays, sample_rmax, precision_runs):
@@ -145,6 +152,12 @@ def bench_uneven(args):
             wall = time.perf_counter() - t0
             per_bucket = []
             for idx, p in plans:
+                # the whole job: every ray has stored all its outputs
+                tll = p.read_outputs()[0]
+                nvr = m._rays["nvr"][idx]
+                written = np.sum(np.isfinite(tll) & (tll < 399.0), axis=1)
+                if np.any(written < nvr):
+                    raise RuntimeError(f"uneven {prec}: {int(np.sum(written < nvr))} rays left unmarched")
                 ms, n = p.march_time()
                 v, r, l = p.describe(0)
                 cs = float(np.sum(m._rays["nz"][idx].astype(np.float64) * m._rays["nsteps"][idx]))
