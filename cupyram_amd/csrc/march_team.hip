@@ -696,8 +696,11 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
 }  // namespace
 
 // Shuffle mode: one 64-lane team per wavefront, four per 256-thread block.
+// At least two waves per SIMD (256 VGPRs): a lone wave cannot hide the
+// latency of its shuffles and reciprocals (measured 1.6x on 8- and 12-row
+// lanes of complex64 against the 1-wave build, which took up to 314 VGPRs).
 template <class R, int M, bool DS, bool DX>
-__global__ void __launch_bounds__(256) k_march_team_w(MarchArgs a, int s0, int s1) {
+__global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, int s1) {
   const int team = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);

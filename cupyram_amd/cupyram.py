@@ -473,12 +473,19 @@ class CuPyRAM:
         # deepest buckets first: their rays have the longest dependent chain
         # per step (fewest rays, largest columns), so they bound the job's end
         # unless they start at once; the wide shallow buckets fill around them
+        # Within a bucket the rays go longest first (most range steps): a
+        # block of teams (or a wavefront of thread-march lanes) holds its CU
+        # slots until its longest ray ends, so rays of similar length share
+        # blocks, and the long ones start at once instead of in the tail.
+        nsteps = self._rays["nsteps"]
         out = []
         for sh, ids in sorted(groups.items(), key=lambda kv: (-int(nz[kv[1]].max()), -len(kv[1]))):
+            ids = numpy.array(ids)
+            ids = ids[numpy.argsort(-nsteps[ids], kind="stable")]
             if sh == (0, 0):
-                out.append((numpy.array(ids), None, "thread"))
+                out.append((ids, None, "thread"))
             else:
-                out.append((numpy.array(ids), sh, "team"))
+                out.append((ids, sh, "team"))
         return out
 
     def run(self):
