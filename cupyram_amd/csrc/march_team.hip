@@ -743,10 +743,11 @@ bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? r
 bool team_rows_elim_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 40 : rows >= 16; }
 
 // Rows per lane built for each precision.  Between the powers of two, 20, 21
-// and 28 (complex64) and 10 (complex128) let a column fill its lanes with few
-// padding rows (nz = 5303: 21 x 256 = 5376 rows instead of 24 x 256 = 6144).
+// and 28 (complex64) and 10 and 11 (complex128) let a column fill its lanes
+// with few padding rows (nz = 5303: 21 x 256 = 5376 rows instead of
+// 24 x 256 = 6144), or fill all eight waves of a CU (11 x 512 lanes).
 constexpr int kRows64[] = {4, 8, 12, 16, 20, 21, 24, 28, 32, 40, 48, 64};
-constexpr int kRows128[] = {2, 4, 6, 8, 10, 12, 16, 24, 32, 40};
+constexpr int kRows128[] = {2, 4, 6, 8, 10, 11, 12, 16, 24, 32, 40};
 
 bool team_shape_supported(int precision, int rows, int lanes) {
   const bool lanes_ok = lanes >= 64 && lanes <= 512 && lanes % 64 == 0;
@@ -838,6 +839,8 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
         return launch_rows<double, 8, true>(a, lanes, s0, s1, st);
       case 10:
         return launch_rows<double, 10, true>(a, lanes, s0, s1, st);
+      case 11:
+        return launch_rows<double, 11, true>(a, lanes, s0, s1, st);
       case 12:
         return launch_rows<double, 12, true>(a, lanes, s0, s1, st);
       case 16:

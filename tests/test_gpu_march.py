@@ -142,7 +142,8 @@ def test_multi_frequency_matches_single(require_gpu):
                                              ("c64", (40, 64)), ("c64", (48, 64)), ("c64", (64, 64)),
                                              ("c128", (16, 64)), ("c128", (24, 64)), ("c128", (32, 64)),
                                              ("c128", (40, 64)), ("c64", (20, 64)), ("c64", (21, 64)),
-                                             ("c64", (28, 64)), ("c128", (10, 64)), ("c64", (21, 256))])
+                                             ("c64", (28, 64)), ("c128", (10, 64)), ("c128", (11, 64)),
+                                             ("c64", (21, 256)), ("c128", (11, 512))])
 def test_every_team_shape_matches_oracle(require_gpu, precision, shape):
     """Register rows (c64 4..12, c128 2..6), streamed-coefficient rows
     (c64 16..32, c128 8..12) and streamed elimination arrays (c64 40..64,
