@@ -22,7 +22,6 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
 bool team_shape_supported(int precision, int rows, int lanes);
 void team_auto_shape(int precision, int nz, int *rows, int *lanes);
 int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays);
-int64_t team_xbuf_bytes(int precision, int rows, int lanes, int nrays);
 
 namespace {
 
@@ -61,7 +60,6 @@ struct cpr_plan {
   double *tlcell_d = nullptr;
   int32_t *team_rays_d = nullptr;
   char *dbuf_d = nullptr;
-  char *xbuf_d = nullptr;
   int variant = CPR_VARIANT_THREAD;
   int team_rows = 0, team_lanes = 0;
   std::vector<hipEvent_t> events;  // pairs around march launches
@@ -98,7 +96,6 @@ struct cpr_plan {
     a.team_rays = team_rays_d;
     a.team_nrays = nrays;
     a.dbuf = dbuf_d;
-    a.xbuf = xbuf_d;
     return a;
   }
   size_t real_size() const { return cfg.precision == CPR_C64 ? sizeof(float) : sizeof(double); }
@@ -211,7 +208,7 @@ int cpr_plan_destroy(cpr_plan *p) {
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
                   static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d),
                   static_cast<void *>(p->team_rays_d), static_cast<void *>(p->dbuf_d),
-                  static_cast<void *>(p->xbuf_d), static_cast<void *>(p->cells_d),
+                  static_cast<void *>(p->cells_d),
                   static_cast<void *>(p->tlcell_d)})
     if (q) (void)hipFree(q);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
@@ -290,8 +287,6 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     if (dalloc(p, &p->team_rays_d, cfg->nrays)) return bail(-1);
     const int64_t db = team_dbuf_bytes(cfg->precision, p->team_rows, p->team_lanes, cfg->nrays);
     if (db > 0 && dalloc(p, &p->dbuf_d, static_cast<size_t>(db))) return bail(-1);
-    const int64_t xb = team_xbuf_bytes(cfg->precision, p->team_rows, p->team_lanes, cfg->nrays);
-    if (xb > 0 && dalloc(p, &p->xbuf_d, static_cast<size_t>(xb))) return bail(-1);
     // on the plan's stream, behind dalloc's clearing of the same buffer: the
     // stream is non-blocking, so a null-stream copy could land before it
     if (hipMemcpyAsync(p->team_rays_d, ids.data(), sizeof(int32_t) * cfg->nrays, hipMemcpyHostToDevice,

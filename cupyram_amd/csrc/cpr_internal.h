@@ -58,5 +58,4 @@ struct MarchArgs {
   const int32_t *team_rays;  // rays handled by this launch (team variant)
   int team_nrays;
   void *dbuf;                // streamed Galerkin coefficients (deep teams)
-  void *xbuf;                // streamed elimination arrays (deepest teams)
 };

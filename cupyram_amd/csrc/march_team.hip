@@ -23,6 +23,7 @@
 // (outpt.py:11-58) are fused in the same persistent loop over range steps.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <type_traits>
 
 #include "march_common.h"
@@ -229,10 +230,11 @@ struct LaneCol {
 };
 
 // Elimination arrays of a lane: c' (then C) and the spike w (then W) of
-// every row, in VGPRs (DX = false) or in a global buffer laid out
-// [ray][row k][lane][2] (DX = true: only the field stays in VGPRs, which
-// lets one lane hold 40-64 complex64 rows of the deepest columns).
-template <class R, int M, bool DX>
+// every row.  They are register arrays at every row count: for the deepest
+// lanes (complex64 40-64 rows, complex128 16-40) the compiler spills part of
+// them to private memory, which measured 1.05-1.8x faster than streaming
+// them through a global per-team buffer.
+template <class R, int M>
 struct Elim {
   cx<R> cw[M], ww[M];
   __device__ cx<R> c(int k) const { return cw[k]; }
@@ -240,18 +242,6 @@ struct Elim {
   __device__ void set(int k, cx<R> cv, cx<R> wv) {
     cw[k] = cv;
     ww[k] = wv;
-  }
-};
-template <class R, int M>
-struct Elim<R, M, true> {
-  cx<R> *ray;  // team-uniform: row k of lane t at ray[2 * (k * T + t)] (c), + 1 (w)
-  int T, t;
-  __device__ cx<R> c(int k) const { return (ray + 2 * k * T)[2 * t]; }
-  __device__ cx<R> w(int k) const { return (ray + 2 * k * T)[2 * t + 1]; }
-  __device__ void set(int k, cx<R> cv, cx<R> wv) {
-    cx<R> *row = ray + 2 * k * T;
-    row[2 * t] = cv;
-    row[2 * t + 1] = wv;
   }
 };
 
@@ -449,8 +439,8 @@ __device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
 // bytes.  Rows past nz (k >= mv) get a zero reciprocal pivot, so c', w and d'
 // vanish there and stay zero through the upward sweep and the back
 // substitution.
-template <bool REAL, class R, int M, bool DS, bool DX>
-__device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M, DX> &X, const Coefs<R, M, DS> &C, cx<R> q,
+template <bool REAL, class R, int M, bool DS>
+__device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coefs<R, M, DS> &C, cx<R> q,
                                       cx<R> q4, cx<R> rho, cx<R> um, cx<R> u_next, int mv) {
   cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(0, 0), dprev = cxm<R>(0, 0);
   using Row = std::conditional_t<REAL, CoefHalf<R>, CoefRow<R>>;
@@ -500,7 +490,7 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M, DX> &X, const
   }
 }
 
-template <class R, int M, int TW, bool DS, bool DX>
+template <class R, int M, int TW, bool DS>
 __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *lds, int s0, int s1) {
   const cpr_ray P = make_uniform(a.rays[b]);
   const cpr_env E = make_uniform(a.envs[b]);
@@ -513,12 +503,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   mv = mv < 0 ? 0 : (mv > M ? M : mv);
   LaneCol<R, M> L;
   Coefs<R, M, DS> C;
-  Elim<R, M, DX> X;
-  if constexpr (DX) {
-    X.ray = static_cast<cx<R> *>(a.xbuf) + 2 * static_cast<int64_t>(b) * M * T;
-    X.T = T;
-    X.t = t;
-  }
+  Elim<R, M> X;
   MedCache<M> mc;
   const double cfact = 0.5 / (P.dz * P.dz);
   if constexpr (DS) {
@@ -699,55 +684,54 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
 // At least two waves per SIMD (256 VGPRs): a lone wave cannot hide the
 // latency of its shuffles and reciprocals (measured 1.6x on 8- and 12-row
 // lanes of complex64 against the 1-wave build, which took up to 314 VGPRs).
-template <class R, int M, bool DS, bool DX>
+template <class R, int M, bool DS>
 __global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, int s1) {
   const int team = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 64, DS, DX>(a, b, threadIdx.x % 64, 64, nullptr, s0, s1);
+  march_team_body<R, M, 64, DS>(a, b, threadIdx.x % 64, 64, nullptr, s0, s1);
 }
 
 // LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64).
-template <class R, int M, bool DS, bool DX>
+template <class R, int M, bool DS>
 __global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s1) {
   extern __shared__ char smem[];
   const int team = blockIdx.x;
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 0, DS, DX>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<cx<R> *>(smem), s0, s1);
+  march_team_body<R, M, 0, DS>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<cx<R> *>(smem), s0, s1);
 }
 
 namespace {
 
-template <class R, int M, bool DS, bool DX = false>
+template <class R, int M, bool DS>
 int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
   if (lanes == 64) {
     const int grid = (a.team_nrays + 3) / 4;
-    hipLaunchKernelGGL((k_march_team_w<R, M, DS, DX>), dim3(grid), dim3(256), 0, st, a, s0, s1);
+    hipLaunchKernelGGL((k_march_team_w<R, M, DS>), dim3(grid), dim3(256), 0, st, a, s0, s1);
   } else {
     if (lanes % 64 || lanes > 512) return -1;
     const size_t lds = team_lds_bytes<R>(lanes);
-    hipLaunchKernelGGL((k_march_team_l<R, M, DS, DX>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
+    hipLaunchKernelGGL((k_march_team_l<R, M, DS>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // namespace
 
-// Rows per lane and where the lane keeps its state: complex64 holds
-// everything in VGPRs up to 12 rows, streams the Galerkin coefficients at 16
-// to 32 rows, and also the elimination arrays at 40, 48 and 64 rows;
-// complex128 likewise at up to 6, 8-12 and 16-40 rows (40 x 512 lanes hold
-// the deepest columns, 20,480 rows).
+// Rows per lane and where the lane keeps its state: the field and the
+// elimination arrays are always register arrays; complex64 keeps the Galerkin
+// coefficients in VGPRs up to 12 rows and streams them from a per-team buffer
+// from 16 rows on, complex128 up to 6 and from 8 rows on (40 x 512 lanes hold
+// the deepest columns, 20,480 rows; complex64 64 x 512 hold 32,768).
 bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 16 : rows >= 8; }
-bool team_rows_elim_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 40 : rows >= 16; }
 
 // Rows per lane built for each precision.  Between the powers of two, 20, 21
 // and 28 (complex64) and 10 and 11 (complex128) let a column fill its lanes
 // with few padding rows (nz = 5303: 21 x 256 = 5376 rows instead of
 // 24 x 256 = 6144), or fill all eight waves of a CU (11 x 512 lanes).
 constexpr int kRows64[] = {4, 8, 12, 16, 20, 21, 24, 28, 32, 40, 48, 64};
-constexpr int kRows128[] = {2, 4, 6, 8, 10, 11, 12, 16, 24, 32, 40};
+constexpr int kRows128[] = {2, 4, 6, 8, 10, 11, 12, 14, 16, 20, 24, 32, 40};
 
 bool team_shape_supported(int precision, int rows, int lanes) {
   const bool lanes_ok = lanes >= 64 && lanes <= 512 && lanes % 64 == 0;
@@ -773,9 +757,12 @@ void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
     int l = (nz + m - 1) / m;
     l = l <= 64 ? 64 : (l + 63) / 64 * 64;
     if (l > 512) continue;
-    // padded row work, a reduced-system term per lane, and premiums for
-    // streaming coefficients and elimination arrays
-    const double prem = team_rows_elim_stream(precision, m) ? 1.6 : (team_rows_stream(precision, m) ? 1.15 : 1.0);
+    // padded row work, a reduced-system term per lane, a premium for
+    // streamed coefficients, and one for the register spills of lanes past
+    // 24 (complex64) or 12 (complex128) rows, fitted to measured rates
+    const int m_fit = precision == CPR_C64 ? 24 : 12;
+    double prem = team_rows_stream(precision, m) ? 1.15 : 1.0;
+    if (m > m_fit) prem *= std::pow(static_cast<double>(m) / m_fit, 1.2);
     const double cost = static_cast<double>(l) * m * prem + 8.0 * l;
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
@@ -793,11 +780,6 @@ int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays) {
   return static_cast<int64_t>(nrays) * rows * lanes * row_bytes;
 }
 
-int64_t team_xbuf_bytes(int precision, int rows, int lanes, int nrays) {
-  if (!team_rows_elim_stream(precision, rows)) return 0;
-  const int64_t cx_bytes = precision == CPR_C64 ? sizeof(cx<float>) : sizeof(cx<double>);
-  return static_cast<int64_t>(nrays) * rows * lanes * 2 * cx_bytes;
-}
 
 int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, int s1, hipStream_t st) {
   if (precision == CPR_C64) {
@@ -821,11 +803,11 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
       case 32:
         return launch_rows<float, 32, true>(a, lanes, s0, s1, st);
       case 40:
-        return launch_rows<float, 40, true, true>(a, lanes, s0, s1, st);
+        return launch_rows<float, 40, true>(a, lanes, s0, s1, st);
       case 48:
-        return launch_rows<float, 48, true, true>(a, lanes, s0, s1, st);
+        return launch_rows<float, 48, true>(a, lanes, s0, s1, st);
       case 64:
-        return launch_rows<float, 64, true, true>(a, lanes, s0, s1, st);
+        return launch_rows<float, 64, true>(a, lanes, s0, s1, st);
     }
   } else {
     switch (rows) {
@@ -843,14 +825,18 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
         return launch_rows<double, 11, true>(a, lanes, s0, s1, st);
       case 12:
         return launch_rows<double, 12, true>(a, lanes, s0, s1, st);
+      case 14:
+        return launch_rows<double, 14, true>(a, lanes, s0, s1, st);
       case 16:
-        return launch_rows<double, 16, true, true>(a, lanes, s0, s1, st);
+        return launch_rows<double, 16, true>(a, lanes, s0, s1, st);
+      case 20:
+        return launch_rows<double, 20, true>(a, lanes, s0, s1, st);
       case 24:
-        return launch_rows<double, 24, true, true>(a, lanes, s0, s1, st);
+        return launch_rows<double, 24, true>(a, lanes, s0, s1, st);
       case 32:
-        return launch_rows<double, 32, true, true>(a, lanes, s0, s1, st);
+        return launch_rows<double, 32, true>(a, lanes, s0, s1, st);
       case 40:
-        return launch_rows<double, 40, true, true>(a, lanes, s0, s1, st);
+        return launch_rows<double, 40, true>(a, lanes, s0, s1, st);
     }
   }
   return -1;
