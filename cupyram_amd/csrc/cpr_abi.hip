@@ -23,6 +23,27 @@ bool team_shape_supported(int precision, int rows, int lanes);
 void team_auto_shape(int precision, int nz, int *rows, int *lanes);
 int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays);
 
+// Ray-major gather of a depth-major output array, for rays b0 .. b0+nb-1:
+// source element (t, i, b) at (t * nz + i) * ld + b, destination element
+// (b - b0, i, t) at ((b - b0) * nz + i) * nt + t, of type T (double for TL,
+// double2 for a complex pressure).  32 x 32 tiles of (t, b) pass through LDS
+// so that the reads (along b) and the writes (along t) are both coalesced.
+template <class T>
+__global__ void __launch_bounds__(256) k_gather_rays(const T *src, T *dst, int64_t ld, int nt, int nz, int b0, int nb) {
+  __shared__ T tile[32][33];
+  const int tb = blockIdx.x * 32, tt = blockIdx.y * 32, i = blockIdx.z;
+  const int lx = threadIdx.x % 32, ly = threadIdx.x / 32;
+  for (int r = ly; r < 32; r += 8) {
+    const int t = tt + r, b = tb + lx;
+    if (t < nt && b < nb) tile[r][lx] = src[(static_cast<int64_t>(t) * nz + i) * ld + b0 + b];
+  }
+  __syncthreads();
+  for (int r = ly; r < 32; r += 8) {
+    const int b = tb + r, t = tt + lx;
+    if (t < nt && b < nb) dst[(static_cast<int64_t>(b) * nz + i) * nt + t] = tile[lx][r];
+  }
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -60,6 +81,8 @@ struct cpr_plan {
   double *tlcell_d = nullptr;
   int32_t *team_rays_d = nullptr;
   char *dbuf_d = nullptr;
+  double *stage_d = nullptr;  // staging buffer of the ray-major output reads
+  int64_t stage_n = 0;        // its size in doubles
   int variant = CPR_VARIANT_THREAD;
   int team_rows = 0, team_lanes = 0;
   std::vector<hipEvent_t> events;  // pairs around march launches
@@ -208,6 +231,7 @@ int cpr_plan_destroy(cpr_plan *p) {
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
                   static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d),
                   static_cast<void *>(p->team_rays_d), static_cast<void *>(p->dbuf_d),
+                  static_cast<void *>(p->stage_d),
                   static_cast<void *>(p->cells_d),
                   static_cast<void *>(p->tlcell_d)})
     if (q) (void)hipFree(q);
@@ -443,72 +467,74 @@ int cpr_march_time(cpr_plan *p, double *ms, int *nlaunch) {
   return 0;
 }
 
-int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double *cpg) {
-  if (!p) return fail("null plan");
+namespace {
+
+constexpr int64_t kStageDoubles = int64_t(4) << 20;  // 32 MiB
+
+// Copies the depth-major array src ([nt][nz][ld] elements of E doubles) to
+// the ray-major host array dst ([nrays][nz][nt]): the rays are gathered on the
+// device in chunks that fill the staging buffer, and each chunk lands in one
+// contiguous span of dst.
+int read_ray_major(cpr_plan *p, const double *src, double *dst, int nt, int nz, int E) {
+  const int64_t per_ray = static_cast<int64_t>(nt) * nz * E;
+  if (per_ray == 0 || p->nrays == 0) return 0;
+  if (nz > 65535 || (nt + 31) / 32 > 65535) return fail("output grid too large for the gather");
+  const int64_t cap = std::max(kStageDoubles, per_ray);  // at least one ray per chunk
+  if (p->stage_n < cap) {
+    if (p->stage_d) HIPCHK(hipFree(p->stage_d));
+    p->stage_d = nullptr;
+    p->stage_n = 0;
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&p->stage_d), cap * sizeof(double)));
+    p->stage_n = cap;
+  }
+  const int chunk = static_cast<int>(std::min<int64_t>(p->nrays, p->stage_n / per_ray));
+  for (int b0 = 0; b0 < p->nrays; b0 += chunk) {
+    const int nb = std::min(chunk, p->nrays - b0);
+    const dim3 grid((nb + 31) / 32, (nt + 31) / 32, nz);
+    if (E == 1)
+      hipLaunchKernelGGL(k_gather_rays<double>, grid, dim3(256), 0, p->stream, src, p->stage_d, p->ld, nt, nz, b0, nb);
+    else
+      hipLaunchKernelGGL(k_gather_rays<double2>, grid, dim3(256), 0, p->stream, reinterpret_cast<const double2 *>(src),
+                         reinterpret_cast<double2 *>(p->stage_d), p->ld, nt, nz, b0, nb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst + b0 * per_ray, p->stage_d, nb * per_ray * sizeof(double), hipMemcpyDeviceToHost,
+                          p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  }
+  return 0;
+}
+
+int tl_post(cpr_plan *p) {
   if (p->cfg.nvr_max > 0) {
     const int block = 64;
     hipLaunchKernelGGL(k_tl_post, dim3((p->nrays + block - 1) / block, p->cfg.nvr_max), dim3(block), 0, p->stream,
                        p->args());
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipStreamSynchronize(p->stream));
-  const int64_t ld = p->ld, nvr = p->cfg.nvr_max, nvz = p->cfg.nvz_max, n = p->nrays;
-  std::vector<double> buf;
-  auto fetch = [&](const double *src, size_t count) -> int {
-    buf.resize(count);
-    HIPCHK(hipMemcpyAsync(buf.data(), src, count * sizeof(double), hipMemcpyDeviceToHost, p->stream));
-    HIPCHK(hipStreamSynchronize(p->stream));
-    return 0;
-  };
-  if (tll) {
-    if (fetch(p->tll_d, nvr * ld)) return -1;
-    for (int64_t b = 0; b < n; ++b)
-      for (int64_t t = 0; t < nvr; ++t) tll[b * nvr + t] = buf[t * ld + b];
-  }
-  if (cpl) {
-    if (fetch(p->cpl_d, 2 * nvr * ld)) return -1;
-    for (int64_t b = 0; b < n; ++b)
-      for (int64_t t = 0; t < nvr; ++t) {
-        cpl[2 * (b * nvr + t)] = buf[2 * (t * ld + b)];
-        cpl[2 * (b * nvr + t) + 1] = buf[2 * (t * ld + b) + 1];
-      }
-  }
+  return 0;
+}
+
+}  // namespace
+
+int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double *cpg) {
+  if (!p) return fail("null plan");
   if ((tlg || cpg) && !p->cfg.compute_grids) return fail("plan was created without grids");
-  if (tlg) {
-    if (fetch(p->tlg_d, nvr * nvz * ld)) return -1;
-    for (int64_t b = 0; b < n; ++b)
-      for (int64_t i = 0; i < nvz; ++i)
-        for (int64_t t = 0; t < nvr; ++t) tlg[(b * nvz + i) * nvr + t] = buf[(t * nvz + i) * ld + b];
-  }
-  if (cpg) {
-    if (fetch(p->cpg_d, 2 * nvr * nvz * ld)) return -1;
-    for (int64_t b = 0; b < n; ++b)
-      for (int64_t i = 0; i < nvz; ++i)
-        for (int64_t t = 0; t < nvr; ++t) {
-          cpg[2 * ((b * nvz + i) * nvr + t)] = buf[2 * ((t * nvz + i) * ld + b)];
-          cpg[2 * ((b * nvz + i) * nvr + t) + 1] = buf[2 * ((t * nvz + i) * ld + b) + 1];
-        }
-  }
+  if (tl_post(p)) return -1;
+  const int nvr = p->cfg.nvr_max, nvz = p->cfg.nvz_max;
+  if (tll && read_ray_major(p, p->tll_d, tll, nvr, 1, 1)) return -1;
+  if (cpl && read_ray_major(p, p->cpl_d, cpl, nvr, 1, 2)) return -1;
+  if (tlg && read_ray_major(p, p->tlg_d, tlg, nvr, nvz, 1)) return -1;
+  if (cpg && read_ray_major(p, p->cpg_d, cpg, nvr, nvz, 2)) return -1;
+  HIPCHK(hipStreamSynchronize(p->stream));
   return 0;
 }
 
 int cpr_read_cells(cpr_plan *p, double *tlc) {
   if (!p || !tlc) return fail("null argument");
   if (!p->cells_d) return fail("plan was created without compute_cells");
-  if (p->cfg.nvr_max > 0) {
-    const int block = 64;
-    hipLaunchKernelGGL(k_tl_post, dim3((p->nrays + block - 1) / block, p->cfg.nvr_max), dim3(block), 0, p->stream,
-                       p->args());
-    HIPCHK(hipGetLastError());
-  }
+  if (tl_post(p)) return -1;
+  if (read_ray_major(p, p->tlcell_d, tlc, p->cfg.nvr_max, p->cfg.nvz_max, 1)) return -1;
   HIPCHK(hipStreamSynchronize(p->stream));
-  const int64_t ld = p->ld, nvr = p->cfg.nvr_max, nvz = p->cfg.nvz_max, n = p->nrays;
-  std::vector<double> buf(static_cast<size_t>(nvr * nvz * ld));
-  HIPCHK(hipMemcpyAsync(buf.data(), p->tlcell_d, buf.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
-  HIPCHK(hipStreamSynchronize(p->stream));
-  for (int64_t b = 0; b < n; ++b)
-    for (int64_t i = 0; i < nvz; ++i)
-      for (int64_t t = 0; t < nvr; ++t) tlc[(b * nvz + i) * nvr + t] = buf[(t * nvz + i) * ld + b];
   return 0;
 }
 

@@ -268,3 +268,28 @@ def test_many_buckets_every_ray_marched(require_gpu):
     err = np.sqrt(np.nanmean((tl["c64"] - tl["c128"]) ** 2, axis=(1, 2)))
     assert np.all(np.isfinite(err))
     assert err.max() <= 0.5, (int(np.argmax(err)), float(err.max()))
+
+
+def test_outputs_read_in_chunks(require_gpu):
+    """40 rays whose pressure grids (1000 ranges x 400 depths) fill several
+    32 MiB staging chunks of the ray-major output read: every output array
+    equals, ray by ray and bitwise, that of the same rays in batches of 10."""
+    n = 40
+    base = envs.pekeris_batching()
+    cb = np.linspace(1600.0, 1800.0, n)
+    lists = {k: [base[k]] * n for k in ("z_ss", "rp_ss", "cw", "z_sb", "rp_sb", "rhob", "attn", "rbzb")}
+    lists["cb"] = [np.array([[c]]) for c in cb]
+
+    def run(sel):
+        kw = dict(freq=base["freq"], zs=base["zs"], zr=base["zr"], rmax=5000.0, dr=5.0, dz=0.5, zmplt=200.0,
+                  compute_grids=True, batch_size=len(sel), precision="c64")
+        kw.update({k: [v[i] for i in sel] for k, v in lists.items()})
+        return CuPyRAM(**kw).run()
+
+    full = run(range(n))
+    per_ray = 2 * int(np.prod(np.shape(full["CP Grid"])[1:]))   # doubles of one ray's pressure grid
+    assert per_ray * n > 3 * (4 << 20)                          # more than three 32 MiB chunks
+    for a in range(0, n, 10):
+        part = run(range(a, a + 10))
+        for key in ("TL Line", "CP Line", "TL Grid", "CP Grid"):
+            np.testing.assert_array_equal(np.asarray(full[key])[a:a + 10], np.asarray(part[key]), err_msg=key)
