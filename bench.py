@@ -230,7 +230,7 @@ def bench_router(args):
             "config": {"workload": text.format(n=n),
                        "members": len(members), "threshold_db": 3.0, "spot_fraction": 0.01},
             "frac_routed_to_pe": res.frac_routed, "batch_fallback": res.batch_fallback,
-            "seconds": res.seconds, "ensemble_spread_median_db": float(np.median(res.spread_db)),
+            "seconds": res.seconds, "ensemble_spread_median_db": float(np.nanmedian(res.spread_db)),
             "spot_checked": int(len(res.spot)),
             "spot_err_rms_db": float(np.sqrt(np.mean(res.spot_err_db ** 2))) if len(res.spot) else None,
             "tl_err_end_to_end_vs_c128": e2e, "tl_err_accepted_surrogate_vs_c128": sur,

@@ -1,17 +1,17 @@
 """Surrogate-first answering of TL queries with PE fallback.
 
-Every query is first answered by the mean of an ensemble of trained
-networks.  It is routed to the complex64 PE march instead when
+Every query is answered by the mean of an ensemble of trained networks,
+unless
 
 * the ensemble disagrees: the RMS over the output grid of the members'
-  standard deviation exceeds `threshold_db`, or
-* it is drawn for the spot check (a seeded `spot_fraction` of the queries
-  the ensemble accepted) and the PE march finds the ensemble mean more than
-  `threshold_db` RMS away from its own field.
+  standard deviation exceeds `threshold_db`; the query goes to the complex64
+  PE march; or
+* it is drawn for the spot check (a seeded `spot_fraction` of the batch): the
+  PE march answers it and judges the ensemble mean on it.
 
-The PE march judges the surrogate: the spot-checked queries give the
-surrogate's measured error, and when their RMS error exceeds the threshold
-the whole batch falls back to the PE march.
+The spot check runs first.  When the RMS error of the ensemble on the spot
+queries it would have answered exceeds the threshold, the whole batch falls
+back to the PE march and the ensemble is not run on the rest.
 """
 from __future__ import annotations
 
@@ -31,9 +31,9 @@ from .train import field_rms, pe_fields
 class RouterResult:
     tl: torch.Tensor            # [n, nr, nzp] dB, the answer for every query
     routed: np.ndarray          # bool [n], answered by the PE march
-    spread_db: np.ndarray       # ensemble disagreement per query
+    spread_db: np.ndarray       # ensemble disagreement per query (NaN where a fallback skipped it)
     spot: np.ndarray            # indices of the spot-checked queries
-    spot_err_db: np.ndarray     # surrogate error on them, judged by the PE march
+    spot_err_db: np.ndarray     # surrogate error on those the ensemble accepted, judged by the PE march
     batch_fallback: bool
     seconds: dict
 
@@ -88,38 +88,56 @@ class Router:
 
     def run(self, q: Queries) -> RouterResult:
         n = len(q)
+        dev = self.device
         t0 = time.perf_counter()
-        mean, spread = self.ensemble(q)
-        self._sync()
-        t_sur = time.perf_counter() - t0
-        spread_h = spread.cpu().numpy()
-        routed = spread_h > self.threshold
-        accepted = np.nonzero(~routed)[0]
+        t_sur = t_pe = 0.0
+        tl = torch.empty(n, self.dom.nr, self.dom.nzp, device=dev)
+        spread_h = np.full(n, np.nan)
+        routed = np.zeros(n, bool)
         rng = np.random.default_rng(self.seed)
-        n_spot = min(len(accepted), int(math.ceil(self.spot_fraction * len(accepted)))) if len(accepted) else 0
-        spot = np.sort(rng.choice(accepted, size=n_spot, replace=False)) if n_spot else np.zeros(0, np.int64)
-        pe_idx = np.union1d(np.nonzero(routed)[0], spot)
-        t1 = time.perf_counter()
-        tl = mean
+        n_spot = min(n, int(math.ceil(self.spot_fraction * n)))
+        spot = np.sort(rng.choice(n, size=n_spot, replace=False)) if n_spot else np.zeros(0, np.int64)
         spot_err = np.zeros(0)
         batch_fallback = False
-        if len(pe_idx):
-            pe_tl, _, _ = self.pe(self.dom, q.take(pe_idx), self.precision, log=self.log)
-            pos = np.searchsorted(pe_idx, spot)
-            if len(spot):
-                spot_err = field_rms(self.dom, mean[torch.as_tensor(spot, device=self.device)],
-                                     pe_tl[torch.as_tensor(pos, device=self.device)]).cpu().numpy()
-            tl[torch.as_tensor(pe_idx, device=self.device)] = pe_tl
-            routed[spot[spot_err > self.threshold]] = True
+        if n_spot:
+            # 1. spot check: ensemble and PE march on the sample
+            t = time.perf_counter()
+            mean_s, spread_s = self.ensemble(q.take(spot))
+            self._sync()
+            t_sur += time.perf_counter() - t
+            t = time.perf_counter()
+            pe_s, _, _ = self.pe(self.dom, q.take(spot), self.precision, log=self.log)
+            self._sync()
+            t_pe += time.perf_counter() - t
+            spread_h[spot] = spread_s.cpu().numpy()
+            acc = spread_h[spot] <= self.threshold
+            err = field_rms(self.dom, mean_s, pe_s).cpu().numpy()
+            spot_err = err[acc]
+            tl[torch.as_tensor(spot, device=dev)] = pe_s
+            routed[spot] = ~acc | (err > self.threshold)
             batch_fallback = bool(len(spot_err) and np.sqrt(np.mean(spot_err ** 2)) > self.threshold)
+        rest = np.setdiff1d(np.arange(n), spot)
+        if len(rest):
+            if batch_fallback:
+                pe_idx = rest
+            else:
+                # 2. the ensemble answers the rest, except where it disagrees
+                t = time.perf_counter()
+                mean_r, spread_r = self.ensemble(q.take(rest))
+                self._sync()
+                t_sur += time.perf_counter() - t
+                spread_h[rest] = spread_r.cpu().numpy()
+                tl[torch.as_tensor(rest, device=dev)] = mean_r
+                pe_idx = rest[spread_h[rest] > self.threshold]
+            if len(pe_idx):
+                t = time.perf_counter()
+                pe_r, _, _ = self.pe(self.dom, q.take(pe_idx), self.precision, log=self.log)
+                tl[torch.as_tensor(pe_idx, device=dev)] = pe_r
+                self._sync()
+                t_pe += time.perf_counter() - t
+                routed[pe_idx] = True
         if batch_fallback:
-            rest = np.nonzero(~np.isin(np.arange(n), pe_idx))[0]
-            if len(rest):
-                rest_tl, _, _ = self.pe(self.dom, q.take(rest), self.precision, log=self.log)
-                tl[torch.as_tensor(rest, device=self.device)] = rest_tl
             routed[:] = True
-        self._sync()
-        t_pe = time.perf_counter() - t1
         return RouterResult(tl=tl, routed=routed, spread_db=spread_h, spot=spot, spot_err_db=spot_err,
                             batch_fallback=batch_fallback,
                             seconds={"surrogate": t_sur, "pe": t_pe, "total": time.perf_counter() - t0})

@@ -50,7 +50,7 @@ def test_agreeing_ensemble_is_accepted_and_spot_checked():
     r = Router(DOM, [Member() for _ in range(4)], pe=make_pe(0.5, calls), device="cpu", batch=64)
     res = r.run(q)
     assert not res.routed.any() and not res.batch_fallback
-    assert len(res.spot) == 3 and calls == [3]                 # 1 % of the accepted queries
+    assert len(res.spot) == 3 and calls == [3]                 # 1 % of the queries
     np.testing.assert_allclose(res.spot_err_db, 0.5, atol=0.02)
     others = np.setdiff1d(np.arange(300), res.spot)
     t = truth(q)
@@ -71,7 +71,9 @@ def test_disagreement_above_3db_routes_to_pe():
     assert np.all(res.spread_db[hi] > 3.0) and np.all(res.spread_db[lo] < 1e-3)
     t = truth(q)
     assert torch.allclose(res.tl[torch.as_tensor(np.nonzero(hi)[0])], t[torch.as_tensor(np.nonzero(hi)[0])])
-    assert set(res.spot).isdisjoint(np.nonzero(res.spread_db > 3.0)[0])
+    # the spot check judges the ensemble only where it would have answered
+    assert len(res.spot_err_db) == int(np.sum(res.spread_db[res.spot] <= 3.0))
+    assert not res.batch_fallback
 
 
 def test_biased_ensemble_fails_the_spot_check_and_the_batch_falls_back():
@@ -82,7 +84,8 @@ def test_biased_ensemble_fails_the_spot_check_and_the_batch_falls_back():
     res = r.run(q)
     assert res.batch_fallback and res.routed.all()
     assert np.all(res.spot_err_db > 3.0)
-    assert sum(calls) == 400
+    assert sum(calls) == 400 and calls[0] == 4                 # the spot sample first, then the rest
+    assert np.isnan(res.spread_db).sum() == 396                # the ensemble skipped the rest
     assert torch.allclose(res.tl, truth(q))
 
 
