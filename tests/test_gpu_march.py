@@ -293,3 +293,28 @@ def test_outputs_read_in_chunks(require_gpu):
         part = run(range(a, a + 10))
         for key in ("TL Line", "CP Line", "TL Grid", "CP Grid"):
             np.testing.assert_array_equal(np.asarray(full[key])[a:a + 10], np.asarray(part[key]), err_msg=key)
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", 0.1), ("c64", 0.5)])
+def test_munk_full_size_sample(require_gpu, precision, tol):
+    """Eight seeded rays of the 10,000-ray Munk workload at its full size
+    (nz 5303, 50 km, 4,999 range steps; the headline team shapes) against the
+    FP64 oracle run over the same range: TL line RMS within the gate."""
+    from cupyram_amd import workloads
+    from oracle import ram_oracle as O
+    kw = workloads.munk_batch(nrays=10000)
+    pick = np.sort(np.random.default_rng(21).choice(10000, size=8, replace=False))
+    for k in ("cb", "rhob", "attn", "rbzb"):
+        if isinstance(kw[k], list):
+            kw[k] = [kw[k][i] for i in pick]
+    kw["batch_size"] = len(pick)
+    m = CuPyRAM(**kw, precision=precision)
+    tl = np.asarray(m.run()["TL Line"])
+    assert m.plan_variants[0][1][0] == 2 and int(m._rays["nz"][0]) == 5303
+    ref = O.run_batch_tl([envs.oracle_for(m, e, 0) for e in range(len(pick))], nthreads=min(16, os.cpu_count() or 1))
+    d = np.concatenate([tl[i, :len(ref[i])] - ref[i] for i in range(len(pick))])
+    d = d[np.isfinite(d)]
+    assert len(d) > 8 * 4900
+    rms = float(np.sqrt(np.mean(d ** 2)))
+    print(f"munk full size {precision}: TL line RMS {rms:.3g} dB, max {float(np.max(np.abs(d))):.3g} dB over {len(d)} points")
+    assert rms <= tol, (rms, float(np.max(np.abs(d))))
