@@ -214,3 +214,21 @@ def test_packed_tables_equal_per_ray_valid_slices(case):
             bt = np.vstack([bt, [rmax, bt[-1, 1]]])
         assert rec["n_bt"] == len(bt)
         assert np.array_equal(pick("rbzb", 2 * len(bt)), bt.ravel())
+
+
+def test_buckets_deepest_first_and_longest_rays_first():
+    """Uneven batches: buckets go deepest first, and the rays of a bucket go
+    longest (most range steps) first, stably; every ray lands in one bucket."""
+    from cupyram_amd import workloads
+    m = CuPyRAM(**workloads.uneven_batch(nrays=400, seed=5), precision="c64")
+    m.setup()
+    buckets = m._buckets()
+    nz, ns = m._rays["nz"], m._rays["nsteps"]
+    seen = np.concatenate([idx for idx, _, _ in buckets])
+    assert np.array_equal(np.sort(seen), np.arange(400))
+    deepest = [int(nz[idx].max()) for idx, _, _ in buckets]
+    assert deepest == sorted(deepest, reverse=True)
+    for idx, shape, variant in buckets:
+        assert np.all(np.diff(ns[idx]) <= 0)
+        ties = np.diff(ns[idx]) == 0
+        assert np.all(np.diff(idx)[ties] > 0)
