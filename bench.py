@@ -321,7 +321,7 @@ def main(argv=None):
                     help="first copy 1 GiB device-to-device (torch) so that HBM counters can be scaled")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(rays are independent: shard the ray list across processes)"}))
         return 2
     if args.copy_calibration:

@@ -1,6 +1,6 @@
 """GPU parity of the HIP march against the CPU oracle (complex128 PyRAM
 restatement), through the C ABI.  Tolerances:
-  * complex128 march vs oracle: TL line RMS <= 0.1 dB (north-star gate; the
+  * complex128 march vs oracle: TL line RMS <= 0.1 dB (the precision gate; the
     observed agreement is ~1e-8 dB);
   * complex64 march vs oracle: TL line RMS <= 0.5 dB;
   * RAM v1.5 golden: mean |dTL| <= 0.01 dB (tests/test_reference.py:28) in

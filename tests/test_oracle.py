@@ -260,7 +260,7 @@ def pekeris_tl(ranges, f, D, c1, c2, rho1, rho2, att, zs, zr):
 
 
 def test_pekeris_normal_modes():
-    """BASELINE config 1 (100 m isovelocity water over a 1700 m/s, 1.5 g/cm3,
+    """The Pekeris CPU configuration (100 m isovelocity water over a 1700 m/s, 1.5 g/cm3,
     0.5 dB/lambda bottom; 50 Hz; zs 25 m, zr 30 m; np 8): the oracle's TL line
     follows the normal-mode sum to <= 1 dB RMS over 1-5 km."""
     from cupyram_amd import workloads
