@@ -35,7 +35,7 @@ METRIC = "Mcell-steps/s (nz×range steps) c64 vs c128 at ≤0.5 dB TL err; HBM G
 # HBM bytes per cell-step of the complex64 team march on the 10,000-ray Munk
 # batch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy
 # (factors 2.0 and 1.0); keyed by (precision, team rows, team lanes).
-MEASURED_TRAFFIC = {(64, 21, 256): {"fetch": 77.26, "write": 1.62,
+MEASURED_TRAFFIC = {(64, 21, 256): {"fetch": 77.77, "write": 1.62,
                                      "source": "profiles/team_march_munk10k.md"}}
 
 

