@@ -34,8 +34,10 @@ METRIC = "Mcell-steps/s (nz×range steps) c64 vs c128 at ≤0.5 dB TL err; HBM G
 
 # HBM bytes per cell-step of the complex64 team march on the 10,000-ray Munk
 # batch, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a 1 GiB copy
-# (factors 2.0 and 1.0); keyed by (precision, team rows, team lanes).
-MEASURED_TRAFFIC = {(64, 21, 256): {"fetch": 77.77, "write": 1.62,
+# (factors 2.0 and 1.0), and the VALU-busy fraction from the SQ counters of
+# the same kernel (4 x SQ_ACTIVE_INST_VALU / SIMDs / busy cycles); keyed by
+# (precision, team rows, team lanes).
+MEASURED_TRAFFIC = {(64, 21, 256): {"fetch": 77.77, "write": 1.62, "valu_busy": 0.895,
                                      "source": "profiles/team_march_munk10k.md"}}
 
 
@@ -373,7 +375,8 @@ def main(argv=None):
                      "alg_bytes_per_cell_step": alg_bytes_per_cell_step(np_pade, 64),
                      "kernel_ms_per_launch": avg_launch_s * 1e3,
                      "traffic_source": meas["source"] if meas else None,
-                     "measured_hbm_gbs": traffic / avg_launch_s / 1e9 if traffic else None},
+                     "measured_hbm_gbs": traffic / avg_launch_s / 1e9 if traffic else None,
+                     "valu_busy_measured": meas["valu_busy"] if meas else None},
         "c128": out_c128,
     }
     if out_c128:
