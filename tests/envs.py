@@ -86,6 +86,20 @@ def varying_ranges():
                 rhob=rhob, attn=attn, rbzb=rbzb, rmax=10000.0, dr=50.0, dz=2.0, batch_size=3)
 
 
+def deep_and_shallow():
+    """Two isovelocity columns in one batch: 5000 m at dz = 0.15 m (about
+    35,300 rows, deeper than every register-team shape, so it takes the
+    one-lane-per-ray march) and 200 m (a team shape); dz is given per ray so
+    that each column gets its own depth grid."""
+    depths = [5000.0, 200.0]
+    return dict(freq=100.0, zs=50.0, zr=60.0,
+                z_ss=[np.array([0.0, d]) for d in depths], rp_ss=[np.array([0.0])] * 2,
+                cw=[np.array([[1500.0], [1500.0]])] * 2, z_sb=[np.array([0.0])] * 2, rp_sb=[np.array([0.0])] * 2,
+                cb=[np.array([[1700.0]])] * 2, rhob=[np.array([[1.5]])] * 2, attn=[np.array([[0.5]])] * 2,
+                rbzb=[np.array([[0.0, d], [1000.0, d]]) for d in depths], rmax=200.0, dr=10.0, dz=np.array([0.15, 0.15]),
+                batch_size=2)
+
+
 def oracle_for(model, env_index=0, freq_index=0):
     """Oracle input for ray (env, freq) of a CuPyRAM instance after setup()."""
     from oracle import ram_oracle as O

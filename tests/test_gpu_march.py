@@ -318,3 +318,10 @@ def test_munk_full_size_sample(require_gpu, precision, tol):
     rms = float(np.sqrt(np.mean(d ** 2)))
     print(f"munk full size {precision}: TL line RMS {rms:.3g} dB, max {float(np.max(np.abs(d))):.3g} dB over {len(d)} points")
     assert rms <= tol, (rms, float(np.max(np.abs(d))))
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", 0.1), ("c64", 0.5)])
+def test_column_deeper_than_every_team_shape(require_gpu, precision, tol):
+    """A 35,300-row column (one-lane-per-ray march) and a shallow one (team
+    march) in one batch, each against the oracle."""
+    _compare_with_oracle(envs.deep_and_shallow(), precision, "auto", tol, grids=False)

@@ -232,3 +232,19 @@ def test_buckets_deepest_first_and_longest_rays_first():
         assert np.all(np.diff(ns[idx]) <= 0)
         ties = np.diff(ns[idx]) == 0
         assert np.all(np.diff(idx)[ties] > 0)
+
+
+@pytest.mark.parametrize("precision", ["c64", "c128"])
+def test_column_deeper_than_every_team_shape_takes_the_thread_march(precision):
+    """A column of about 35,300 rows exceeds the largest team shape of both
+    precisions: 'auto' puts it in a one-lane-per-ray bucket beside the team
+    bucket of the shallow ray, and variant='team' refuses the batch."""
+    m = CuPyRAM(**envs.deep_and_shallow(), precision=precision)
+    m.setup()
+    assert int(m._rays["nz"].max()) > 32768
+    kinds = sorted(v for _, _, v in m._buckets())
+    assert kinds == ["team", "thread"]
+    t = CuPyRAM(**envs.deep_and_shallow(), precision=precision, variant="team")
+    t.setup()
+    with pytest.raises(ValueError, match="too deep"):
+        t._buckets()
