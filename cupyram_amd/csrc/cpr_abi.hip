@@ -482,10 +482,12 @@ int read_ray_major(cpr_plan *p, const double *src, double *dst, int nt, int nz, 
   const int64_t cap = std::max(kStageDoubles, per_ray);  // at least one ray per chunk
   if (p->stage_n < cap) {
     if (p->stage_d) HIPCHK(hipFree(p->stage_d));
+    p->bytes -= p->stage_n * static_cast<int64_t>(sizeof(double));
     p->stage_d = nullptr;
     p->stage_n = 0;
     HIPCHK(hipMalloc(reinterpret_cast<void **>(&p->stage_d), cap * sizeof(double)));
     p->stage_n = cap;
+    p->bytes += cap * static_cast<int64_t>(sizeof(double));
   }
   const int chunk = static_cast<int>(std::min<int64_t>(p->nrays, p->stage_n / per_ray));
   for (int b0 = 0; b0 < p->nrays; b0 += chunk) {

@@ -121,7 +121,8 @@ int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double 
 int cpr_read_cells(cpr_plan *p, double *tlc);
 /* Field u after the last march step, [nrays][nz_max+2] complex128. */
 int cpr_read_field(cpr_plan *p, double *u, int nz_max);
-/* Bytes of device memory held by the plan. */
+/* Bytes of device memory held by the plan (the staging buffer of the output
+ * reads included once the first read has allocated it). */
 int64_t cpr_plan_device_bytes(cpr_plan *p);
 /* Team shape (rows per lane, lanes per ray) the library would pick for a
  * column of nz rows; rows = lanes = 0 when no register-resident team holds
