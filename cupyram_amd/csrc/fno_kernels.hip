@@ -26,18 +26,27 @@ namespace {
 
 constexpr int BM = 128, BN = 128, BK = 32, LDP = BK + 8;
 
-__device__ inline float gelu_f(float x) {
-  // tanh form of GELU (Hendrycks & Gimpel 2016)
-  const float k = 0.7978845608028654f;
-  const float u = k * (x + 0.044715f * x * x * x);
-  return 0.5f * x * (1.0f + tanhf(u));
+// tanh form of GELU (Hendrycks & Gimpel 2016), 0.5 x (1 + tanh u) with
+// u = k (x + 0.044715 x^3), written through the identity
+// 0.5 (1 + tanh u) = s(u) = 1 / (1 + e^(-2u)) and evaluated with the hardware
+// exp2 and reciprocal (about 1e-7 relative error, far below the bf16 rounding
+// of the activations; e^(-2u) = inf gives s = 0, e^(-2u) = 0 gives s = 1).
+__device__ inline float gelu_sig(float x, float &x2) {
+  const float k2 = 2.0f * 0.7978845608028654f * 1.4426950408889634f;  // 2 k log2(e)
+  x2 = x * x;
+  const float v = k2 * (x + 0.044715f * x2 * x);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-v));
 }
+__device__ inline float gelu_f(float x) {
+  float x2;
+  return x * gelu_sig(x, x2);
+}
+// d/dx [x s] = s + x s (1 - s) 2k (1 + 3 * 0.044715 x^2)
 __device__ inline float gelu_grad(float x) {
   const float k = 0.7978845608028654f;
-  const float x2 = x * x;
-  const float u = k * (x + 0.044715f * x2 * x);
-  const float t = tanhf(u);
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * x2);
+  float x2;
+  const float s = gelu_sig(x, x2);
+  return s + x * s * (1.0f - s) * 2.0f * k * (1.0f + 3.0f * 0.044715f * x2);
 }
 
 struct GemmArgs {
