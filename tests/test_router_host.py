@@ -94,3 +94,21 @@ def test_router_rejects_bad_configuration():
         Router(DOM, [], device="cpu")
     with pytest.raises(ValueError):
         Router(DOM, [Member()], spot_fraction=1.5, device="cpu")
+
+
+def test_no_spot_check_and_single_query():
+    """spot_fraction 0: the ensemble answers everything and the march is never
+    called; a one-query batch is its own spot sample."""
+    q = D.wedge_queries(50, 14)
+    calls = []
+    r = Router(DOM, [Member() for _ in range(4)], spot_fraction=0.0, pe=make_pe(0.0, calls), device="cpu", batch=64)
+    res = r.run(q)
+    assert calls == [] and len(res.spot) == 0 and not res.routed.any() and not res.batch_fallback
+    assert np.all(np.isfinite(res.spread_db))
+    assert torch.allclose(res.tl, truth(q), atol=0.1)
+    one = q.take(np.array([3]))
+    calls = []
+    r = Router(DOM, [Member() for _ in range(4)], pe=make_pe(0.5, calls), device="cpu", batch=64)
+    res = r.run(one)
+    assert calls == [1] and list(res.spot) == [0] and not res.batch_fallback
+    assert torch.allclose(res.tl, truth(one) - 0.5, atol=1e-6)
