@@ -117,6 +117,7 @@ def cpu_baseline(kw, sample_rays, sample_rmax, precision_runs):
         errs[prec] = {"rms_db": float(np.sqrt(np.mean(d ** 2))), "max_db": float(np.max(np.abs(d))),
                       "frac_gt_0.5db": float(np.mean(np.abs(d) > 0.5))}
     return {"value": cell_steps / t / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "per_core": cell_steps / t / 1e6 / cores,
             "sample": f"{len(pick)} seeded rays of the batch, rmax {sample_rmax:.0f} m "
                       f"({int(m._rays['nsteps'][0])} steps), full nz, FP64 oracle, "
                       f"OpenMP over rays ({cell_steps:.3g} cell-steps in {t:.1f} s)"}, errs
