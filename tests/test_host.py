@@ -248,3 +248,24 @@ def test_column_deeper_than_every_team_shape_takes_the_thread_march(precision):
     t.setup()
     with pytest.raises(ValueError, match="too deep"):
         t._buckets()
+
+
+@pytest.mark.parametrize("precision", [64, 128])
+def test_tuning_table_and_static_shape_model(precision):
+    """The shipped tuning table and the library's static model both give, for
+    every column depth they accept, a built row count with a lane count that
+    is a multiple of 64 (at most 512) covering the column; past the deepest
+    team (complex64 64 x 512 rows, complex128 40 x 512) the static model
+    answers (0, 0), the one-lane-per-ray march."""
+    from cupyram_amd import autotune
+    table = autotune.load_table()
+    assert table.get(str(precision)), "tuning table missing"
+    deepest = 64 * 512 if precision == 64 else 40 * 512
+    for nz in list(range(50, 6000, 37)) + [8000, 12000, 16000, 20480, deepest]:
+        for shape in (autotune.tuned_shape(precision, nz), _abi.team_shape(precision, nz)):
+            if shape is None:
+                continue
+            rows, lanes = shape
+            assert rows in autotune.ROWS[precision], (nz, shape)
+            assert lanes % 64 == 0 and 64 <= lanes <= 512 and rows * lanes >= nz, (nz, shape)
+    assert _abi.team_shape(precision, deepest + 1) == (0, 0)
