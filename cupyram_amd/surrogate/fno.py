@@ -128,12 +128,6 @@ class FNO:
         for l in range(1, self.cfg.layers + 1):
             k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes, self.cfg.width)
 
-    def state_dict(self):
-        d = {n: t.detach().cpu().clone() for n, t in self.params.items()}
-        d["_config"] = torch.tensor([self.cfg.features, self.cfg.width, self.cfg.modes, self.cfg.layers,
-                                     self.cfg.depth_points])
-        return d
-
     def load_state_dict(self, d):
         for n in self.names:
             self.params[n].copy_(d[n].to(self.device, torch.float32))
