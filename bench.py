@@ -123,6 +123,48 @@ def cpu_baseline(kw, sample_rays, sample_rmax, precision_runs):
                       f"OpenMP over rays ({cell_steps:.3g} cell-steps in {t:.1f} s)"}, errs
 
 
+def uneven_cpu_baseline(kw, nz, per_decile=16, sample_rmax=5000.0, seed=11):
+    """Oracle on a sample of the uneven batch stratified by column depth
+    (per_decile seeded rays from each tenth of the rays sorted by nz), with
+    the range cut to sample_rmax so that it finishes in seconds, and the TL
+    error of both GPU marches on the same rays."""
+    from cupyram_amd import CuPyRAM
+    from oracle import ram_oracle as O
+    from tests import envs as E
+    rng = np.random.default_rng(seed)
+    order = np.argsort(nz, kind="stable")
+    pick = np.sort(np.concatenate([rng.choice(part, size=min(per_decile, len(part)), replace=False)
+                                   for part in np.array_split(order, 10)]))
+    skw = dict(kw)
+    for k in ("z_ss", "cw", "cb", "rhob", "attn", "rbzb"):
+        skw[k] = [kw[k][i] for i in pick]
+    for k in ("freq", "dz", "dr"):
+        skw[k] = np.asarray(kw[k])[pick]
+    skw["batch_size"] = len(pick)
+    skw["rmax"] = sample_rmax
+    m = CuPyRAM(**skw)
+    m.setup()
+    rays = [E.oracle_for(m, e, 0) for e in range(len(pick))]
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    tl_ref = O.run_batch_tl(rays, nthreads=cores)
+    t = time.perf_counter() - t0
+    cell_steps = float(np.sum(m._rays["nz"].astype(np.float64) * m._rays["nsteps"]))
+    errs = {}
+    for prec in ("c64", "c128"):
+        g = CuPyRAM(**skw, precision=prec).run()["TL Line"]
+        d = np.concatenate([g[i, :len(tl_ref[i])] - tl_ref[i] for i in range(len(pick))])
+        d = d[np.isfinite(d)]
+        errs[prec] = {"rms_db": float(np.sqrt(np.mean(d ** 2))), "max_db": float(np.max(np.abs(d))),
+                      "frac_gt_0.5db": float(np.mean(np.abs(d) > 0.5))}
+    cb = {"value": cell_steps / t / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+          "per_core": cell_steps / t / 1e6 / cores,
+          "sample": f"{len(pick)} seeded rays stratified by nz ({int(m._rays['nz'].min())}-{int(m._rays['nz'].max())} "
+                    f"rows), rmax {sample_rmax:.0f} m, FP64 oracle, OpenMP over rays "
+                    f"({cell_steps:.3g} cell-steps in {t:.1f} s)"}
+    return cb, errs
+
+
 def bench_uneven(args):
     """The uneven batch (per-ray frequency and grid): the whole job after the
     self-starter, every nz-bucket on its own stream, march launches of
@@ -172,6 +214,7 @@ def bench_uneven(args):
                 p.close()
         cells = float(np.sum(m._rays["nz"].astype(np.float64) * m._rays["nsteps"]))
         out[prec] = {"value": cells / wall / 1e6, "wall_s": wall, "cell_steps": cells, "buckets": per_bucket}
+        nz_all = m._rays["nz"].copy()
     line = {"metric": METRIC, "value": out["c64"]["value"], "unit": "Mcell-steps/s", "n_gpus": 1,
             "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "fp32 (complex64 march; FP64 Pade and self-starter)",
@@ -181,6 +224,12 @@ def bench_uneven(args):
             "c64": out["c64"], "c128": out.get("c128")}
     if "c128" in out:
         line["speedup_c64_vs_c128"] = out["c64"]["value"] / out["c128"]["value"]
+    if not args.no_cpu:
+        log("uneven: CPU oracle on a stratified sample")
+        cb, errs = uneven_cpu_baseline(kw, nz_all)
+        line["cpu_baseline"] = cb
+        line["tl_err_vs_fp64_oracle"] = errs
+        line["speedup_vs_cpu"] = line["value"] / cb["value"]
     print(json.dumps(line), flush=True)
     return 0
 
