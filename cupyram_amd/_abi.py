@@ -67,6 +67,7 @@ _LOCK = threading.Lock()
 _SYMBOLS = {
     "cpr_last_error": ([], C.c_char_p),
     "cpr_device_count": ([C.POINTER(C.c_int)], C.c_int),
+    "cpr_device_info": ([C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "cpr_pade": ([C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
     "cpr_pade_batch": ([C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                         C.c_int], C.c_int),
@@ -142,6 +143,14 @@ def device_count() -> int:
     n = C.c_int(0)
     lib().cpr_device_count(C.byref(n))
     return n.value
+
+
+def device_info(device: int = 0):
+    """(name, free bytes, total bytes) of a HIP device."""
+    buf = C.create_string_buffer(256)
+    free, total = C.c_int64(), C.c_int64()
+    check(lib().cpr_device_info(int(device), buf, len(buf), C.byref(free), C.byref(total)))
+    return buf.value.decode(errors="replace"), free.value, total.value
 
 
 def team_shape(precision: int, nz: int):

@@ -101,16 +101,15 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"])
     args = ap.parse_args(argv)
     freqs = args.frequency or [100.0]
-    import torch  # device properties and free memory
-    if not torch.cuda.is_available():
+    from . import _abi
+    if _abi.device_count() < 1:
         print("no HIP device available", file=sys.stderr)
         return 1
-    props = torch.cuda.get_device_properties(0)
-    free, total = torch.cuda.mem_get_info()
+    dev_name, free, total = _abi.device_info(0)
     print("=" * 70)
     print("CuPyRAM batch scaling benchmark (MI355X march)")
     print("=" * 70)
-    print(f"Device: {props.name}; memory {total / 2**30:.1f} GiB, free {free / 2**30:.1f} GiB")
+    print(f"Device: {dev_name}; memory {total / 2**30:.1f} GiB, free {free / 2**30:.1f} GiB")
     bpr = bytes_per_ray(freqs, args.rmax, args.source_depth, args.receiver_depth, args.max_workers, args.precision)
     sizes = batch_sizes(bpr, free, len(freqs), args.n_steps)
     print(f"Frequencies {freqs} Hz, rmax {args.rmax / 1000:.1f} km, zs {args.source_depth} m, "

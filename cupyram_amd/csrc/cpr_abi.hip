@@ -205,6 +205,29 @@ int cpr_device_count(int *count) {
   return 0;
 }
 
+int cpr_device_info(int device, char *name, int name_len, int64_t *free_bytes, int64_t *total_bytes) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail("cpr_device_info: no such device");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (name && name_len > 0) {
+    std::snprintf(name, static_cast<size_t>(name_len), "%s", prop.name);
+  }
+  if (free_bytes || total_bytes) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    HIPCHK(hipSetDevice(device));
+    size_t fr = 0, to = 0;
+    const hipError_t e = hipMemGetInfo(&fr, &to);
+    HIPCHK(hipSetDevice(cur));
+    HIPCHK(e);
+    if (free_bytes) *free_bytes = static_cast<int64_t>(fr);
+    if (total_bytes) *total_bytes = static_cast<int64_t>(to);
+  }
+  return 0;
+}
+
 int cpr_pade(double freq, double c0, int np_, int ns, double dr, int ip, double *pd1, double *pd2) {
   const int rc = cpr_pade_impl(freq, c0, np_, ns, dr, ip, pd1, pd2);
   if (rc == -1) return fail("Laguerre method not converging");

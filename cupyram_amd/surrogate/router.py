@@ -114,8 +114,10 @@ class Router:
             err = field_rms(self.dom, mean_s, pe_s).cpu().numpy()
             spot_err = err[acc]
             tl[torch.as_tensor(spot, device=dev)] = pe_s
-            routed[spot] = ~acc | (err > self.threshold)
-            batch_fallback = bool(len(spot_err) and np.sqrt(np.mean(spot_err ** 2)) > self.threshold)
+            # comparisons written as ~(x <= threshold) so that a NaN spread or
+            # error (a member that returned NaN) counts as a failure
+            routed[spot] = ~acc | ~(err <= self.threshold)
+            batch_fallback = bool(len(spot_err) and not np.sqrt(np.mean(spot_err ** 2)) <= self.threshold)
         rest = np.setdiff1d(np.arange(n), spot)
         if len(rest):
             if batch_fallback:
@@ -128,7 +130,7 @@ class Router:
                 t_sur += time.perf_counter() - t
                 spread_h[rest] = spread_r.cpu().numpy()
                 tl[torch.as_tensor(rest, device=dev)] = mean_r
-                pe_idx = rest[spread_h[rest] > self.threshold]
+                pe_idx = rest[~(spread_h[rest] <= self.threshold)]
             if len(pe_idx):
                 t = time.perf_counter()
                 pe_r, _, _ = self.pe(self.dom, q.take(pe_idx), self.precision, log=self.log)

@@ -81,6 +81,11 @@ typedef struct cpr_plan cpr_plan;
 
 const char *cpr_last_error(void);
 int cpr_device_count(int *count);
+/* Name (NUL-terminated, truncated to name_len bytes) and free / total device
+ * memory of HIP device `device`; any output may be NULL.  The batch-scaling
+ * CLI's device report and batch sizing; replaces get_gpu_info
+ * (cupyram/benchmark.py:22-40: getDeviceProperties and memGetInfo). */
+int cpr_device_info(int device, char *name, int name_len, int64_t *free_bytes, int64_t *total_bytes);
 
 /* FP64 Pade coefficients; pd1/pd2 are complex128 as interleaved re,im pairs
  * of length 2*np.  Returns -1 if Laguerre fails to converge. */

@@ -532,7 +532,7 @@ int orc_dims(const orc_ray *in, int *nz, int *nvr, int *nvz, int *nsteps) {
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
 static int run_impl(const orc_ray *in, double *tll, double complex *cpl, double *tlg, double complex *cpg,
-                    int *iz_hist, double *tlcell) {
+                    int *iz_hist, double *tlcell, int *trace, double complex *u_final) {
   ray_state s;
   memset(&s, 0, sizeof(s));
   s.in = in;
@@ -628,12 +628,19 @@ static int run_impl(const orc_ray *in, double *tll, double complex *cpl, double 
       goto done;
     }
     if (iz_hist) iz_hist[rn] = s.iz;
+    if (trace) {
+      trace[4 * rn + 0] = s.iz;
+      trace[4 * rn + 1] = s.ss_ind;
+      trace[4 * rn + 2] = s.sb_ind;
+      trace[4 * rn + 3] = s.ns;
+    }
     if (need_profl) ray_profl(&s);
     if (need) ray_matrc(&s, s.np_);
     ray_solve(&s, s.np_);
     s.r = (rn + 2) * in->dr;
     ray_outpt(&s);
   }
+  if (u_final) memcpy(u_final, s.u, sizeof(cplx) * nz2);
   if (tlcell) {
     const double inv_n = 1.0 / ((double)in->ndr * in->ndz);
     for (size_t k = 0; k < (size_t)s.nvz * s.nvr; ++k) tlcell[k] = -10.0 * log10(tlcell[k] * inv_n + 1e-40);
@@ -663,11 +670,15 @@ done:
 
 int orc_run(const orc_ray *in, double *tll, double complex *cpl, double *tlg, double complex *cpg,
             int *iz_hist) {
-  return run_impl(in, tll, cpl, tlg, cpg, iz_hist, NULL);
+  return run_impl(in, tll, cpl, tlg, cpg, iz_hist, NULL, NULL, NULL);
+}
+
+int orc_run_trace(const orc_ray *in, double *tll, int *trace, double complex *u_final) {
+  return run_impl(in, tll, NULL, NULL, NULL, NULL, NULL, trace, u_final);
 }
 
 int orc_run_cells(const orc_ray *in, double *tll, double *tlcell) {
-  return run_impl(in, tll, NULL, NULL, NULL, NULL, tlcell);
+  return run_impl(in, tll, NULL, NULL, NULL, NULL, tlcell, NULL, NULL);
 }
 
 int orc_run_batch(const orc_ray *rays, int nrays, double *const *tll, int nthreads) {

@@ -34,6 +34,9 @@ int orc_run(const orc_ray *in, double *tll, double complex *cpl, double *tlg, do
  * of |f3 u|^2 / r over the fields at ranges (i ndr, (i+1) ndr] dr and rows
  * (j ndz, (j+1) ndz] (the build's coarse TL; no reference counterpart). */
 int orc_run_cells(const orc_ray *in, double *tll, double *tlcell);
+/* TL line, the per-step history trace [nsteps][4] = (iz, ss_ind, sb_ind, ns)
+ * after each range update, and the field u [nz+2] after the last step. */
+int orc_run_trace(const orc_ray *in, double *tll, int *trace, double complex *u_final);
 int orc_run_batch(const orc_ray *rays, int nrays, double *const *tll, int nthreads);
 int orc_profiles(const orc_ray *in, int ss_ind, int sb_ind, double *alpw, double *alpb, double *rhob,
                  double complex *ksqw, double complex *ksqb);

@@ -54,6 +54,7 @@ def lib():
         L.orc_pade.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, dp, dp]
         L.orc_dims.argtypes = [C.POINTER(_Ray)] + [C.POINTER(C.c_int)] * 4
         L.orc_run.argtypes = [C.POINTER(_Ray), dp, dp, dp, dp, C.POINTER(C.c_int)]
+        L.orc_run_trace.argtypes = [C.POINTER(_Ray), dp, C.POINTER(C.c_int), dp]
         L.orc_run_batch.argtypes = [C.POINTER(_Ray), C.c_int, C.POINTER(dp), C.c_int]
         L.orc_run_cells.argtypes = [C.POINTER(_Ray), dp, dp]
         L.orc_profiles.argtypes = [C.POINTER(_Ray), C.c_int, C.c_int, dp, dp, dp, dp, dp]
@@ -176,6 +177,22 @@ def run(ray: OracleRay, grids=True):
             "Depths": np.arange(1, nvz + 1) * ray.dz * ray.ndz,
             "TL Line": tll, "CP Line": cpl, "TL Grid": tlg, "CP Grid": cpg,
             "iz": izh[:nsteps], "nz": nz}
+
+
+def run_trace(ray: OracleRay):
+    """TL line, the per-step history [nsteps, 4] = (iz, ss_ind, sb_ind, ns)
+    after each range update, and the field u [nz+2] after the last step."""
+    nz, nvr, _, nsteps = ray.dims()
+    s = ray.struct()
+    tll = np.zeros(nvr)
+    trace = np.zeros((max(nsteps, 1), 4), np.int32)
+    u = np.zeros(nz + 2, np.complex128)
+    rc = lib().orc_run_trace(C.byref(s), _dp(tll), trace.ctypes.data_as(C.POINTER(C.c_int)), _dp(u))
+    if rc == -1:
+        raise ValueError("Laguerre method not converging")
+    if rc:
+        raise ValueError(f"orc_run_trace failed ({rc})")
+    return {"TL Line": tll, "trace": trace[:nsteps], "u": u, "nz": nz}
 
 
 def run_cells(ray: OracleRay):

@@ -23,6 +23,31 @@ def test_library_exports_every_declared_symbol():
     assert declared <= set(_abi._SYMBOLS)
 
 
+def test_reference_import_names():
+    """The reference's package and submodule names (cupyram/__init__.py:1-3,
+    15-19; the reference's tests import ``from cupyram import CuPyRAM`` and
+    ``from cupyram import CuPyRAM, compute_pade_coefficients``) resolve to
+    this build's objects."""
+    import cupyram
+    from cupyram import AsyncBatcher as A, CuPyRAM as M, compute_pade_coefficients as P
+    from cupyram.batching import AsyncBatcher as A2
+    from cupyram.cupyram import CuPyRAM as M2
+    from cupyram.pade import compute_pade_coefficients as P2
+    assert M is M2 is CuPyRAM and P is P2 is compute_pade_coefficients and A is A2 is AsyncBatcher
+    assert set(cupyram.__all__) == {"CuPyRAM", "compute_pade_coefficients", "AsyncBatcher"}
+    pd1, pd2 = P(freq=100.0, c0=1500.0, np_pade=6, ns=1, dr=50.0, ip=1)
+    o1, o2 = O.pade(100.0, 1500.0, 6, 1, 50.0, 1)
+    np.testing.assert_allclose(pd1, o1, rtol=1e-12)
+    np.testing.assert_allclose(pd2, o2, rtol=1e-12)
+
+
+def test_device_info_without_device_fails_loudly():
+    if _abi.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(_abi.CprError):
+        _abi.device_info(0)
+
+
 def test_plan_create_without_device_fails_loudly():
     if _abi.device_count() > 0:
         pytest.skip("a GPU is visible")
@@ -124,6 +149,32 @@ def test_setup_grid_matches_oracle(name):
     # attributes the reference's tests read after run()/setup()
     assert m.nz == int(m._rays["nz"].max()) and m._dz > 0 and m._dr > 0
     assert len(m.vr) == m._nvr
+
+
+@pytest.mark.parametrize("name", ["ram_reference_case", "pekeris_batching", "pekeris_phase1", "wedge",
+                                  "munk_batching", "pekeris_c1"])
+def test_resolved_parameters_match_independent_defaults(name):
+    """The product's resolved c0, wavelength, zmax, nz, dr, dz, rs and grid
+    counts of a single-ray environment against the oracle's own restatement
+    of the defaults (oracle.resolve_ray: c0 = mean of the first sound-speed
+    column, lambda = c0/f, dr = np 1500/f, dz = 0.1 1500/f, as in
+    cupyram.py:460-608), built from the keyword arguments alone and not from
+    the instance."""
+    from cupyram_amd import workloads
+    kw = workloads.pekeris_c1() if name == "pekeris_c1" else getattr(envs, name)()
+    m = CuPyRAM(**kw)
+    m.setup()
+    ray = O.resolve_ray(**kw)
+    r = m._rays[0]
+    assert r["c0"] == pytest.approx(ray.c0, rel=1e-14)
+    assert r["lyrw_lambda"] == pytest.approx(ray.lyrw * ray.lam, rel=1e-14)
+    assert r["zmax"] == pytest.approx(ray.zmax, rel=1e-14)
+    assert r["dz"] == pytest.approx(ray.dz, rel=1e-15) and r["dr"] == pytest.approx(ray.dr, rel=1e-15)
+    assert r["rs"] == pytest.approx(ray.rs, rel=1e-14)
+    assert r["k0"] == pytest.approx(2 * np.pi * ray.freq / ray.c0, rel=1e-14)
+    nz, nvr, nvz, nsteps = ray.dims()
+    assert (r["nz"], r["nvr"], r["nvz"], r["nsteps"]) == (nz, nvr, nvz, nsteps)
+    assert m._zmplt_e[0] == pytest.approx(ray.zmplt)
 
 
 def test_per_ray_grids():

@@ -313,3 +313,75 @@ def test_cells_are_intensity_means_of_the_fine_grid(ndr, ndz):
             blk = inten[j * ndz:(j + 1) * ndz, i * ndr:(i + 1) * ndr]
             expect[j, i] = -10.0 * np.log10(blk.mean())
     np.testing.assert_allclose(coarse, expect, atol=1e-8)
+
+
+# -- range-update history (profile columns, stability switch) ------------------
+
+def _single(kw, i):
+    return {k: (v[i] if isinstance(v, list) else v) for k, v in kw.items() if k != "batch_size"}
+
+
+@pytest.mark.parametrize("ray_index", [0, 1, 2])
+def test_profile_columns_brute_force(ray_index):
+    """The water and seabed profile columns in use at every step: the last
+    column whose range does not exceed r = (rn+1) dr (step-switched, no range
+    interpolation; updat.py:38-140), found here by a scan over the table."""
+    kw = _single(envs.varying_ranges(), ray_index)
+    ray = O.resolve_ray(**kw)
+    tr = O.run_trace(ray)["trace"]
+    nsteps = tr.shape[0]
+    for col, rp in ((1, ray.rp_ss), (2, ray.rp_sb)):
+        expect = []
+        for rn in range(nsteps):
+            r = (rn + 1) * ray.dr
+            expect.append(max(i for i in range(len(rp)) if i == 0 or rp[i] <= r))
+        assert np.array_equal(tr[:, col], expect), col
+        # every column whose range the march reaches is used
+        assert len(set(expect)) == int(np.sum(rp <= nsteps * ray.dr)) >= 2
+
+
+def _rs_case(**over):
+    from cupyram_amd import workloads
+    kw = workloads.pekeris_c1()
+    kw.update(rmax=600.0, dr=5.0)
+    kw.update(over)
+    return O.resolve_ray(**kw)
+
+
+@pytest.mark.parametrize("rs", [5.0, 212.5, 400.0])
+def test_stability_switch_step_and_factors(rs):
+    """Past r >= rs the march drops the stability constraints (Pade set with
+    ns = 0, cupyram.py:976-999).  Pinned three ways on the range-independent
+    Pekeris waveguide:
+      * the switch step: ns is 1 before the first step with (rn+1) dr >= rs
+        and 0 from it on (brute force);
+      * before the switch the run is bitwise the run without rs;
+      * the step at the switch applies exactly the ns = 0 factors to the
+        field of the step before it (same medium, same bathymetry row), and
+        not the ns = 1 factors."""
+    base = _rs_case()
+    nz, _, _, nsteps = base.dims()
+    k = int(np.ceil(rs / base.dr)) - 1                 # first step with (k+1) dr >= rs
+    assert 0 <= k < nsteps - 1
+    with_rs = _rs_case(rs=rs)
+    tr = O.run_trace(with_rs)["trace"]
+    assert np.array_equal(tr[:, 3], np.where(np.arange(nsteps) < k, 1, 0))
+    a = O.run(base, grids=False)["TL Line"]
+    b = O.run(with_rs, grids=False)["TL Line"]
+    assert np.array_equal(a[:k + 1], b[:k + 1])        # output i is the field after step i-1
+    assert np.max(np.abs(a[k + 2:] - b[k + 2:])) > 1e-6
+    # the field before the switch step (k steps marched) and after it (k+1)
+    before = O.run_trace(_rs_case(rmax=(k + 1) * base.dr))["u"]
+    after = O.run_trace(_rs_case(rmax=(k + 2) * base.dr, rs=rs))["u"]
+    alpw, alpb, rhob, ksqw, ksqb = O.profiles(base)
+    iz = int(np.clip(np.floor(base.rbzb[0, 1] / base.dz), 1, nz - 1))
+    k0 = 2 * np.pi * base.freq / base.c0
+    p1, p2 = O.pade(base.freq, base.c0, base.np_pade, ns=0, dr=base.dr, ip=1)
+    expect = O.apply_factors(nz, iz, base.dz, k0, p1, p2, alpw, alpb, rhob, ksqw, ksqb, before)
+    scale = np.max(np.abs(expect))
+    np.testing.assert_allclose(after[1:nz + 1], expect[1:nz + 1], rtol=0, atol=1e-12 * scale)
+    q1, q2 = O.pade(base.freq, base.c0, base.np_pade, ns=1, dr=base.dr, ip=1)
+    wrong = O.apply_factors(nz, iz, base.dz, k0, q1, q2, alpw, alpb, rhob, ksqw, ksqb, before)
+    # the constraint moves the field by ~1e-7 of its peak after one step,
+    # five orders of magnitude above the agreement with the ns = 0 factors
+    assert np.max(np.abs(wrong[1:nz + 1] - after[1:nz + 1])) > 1e-9 * scale

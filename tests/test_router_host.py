@@ -112,3 +112,48 @@ def test_no_spot_check_and_single_query():
     res = r.run(one)
     assert calls == [1] and list(res.spot) == [0] and not res.batch_fallback
     assert torch.allclose(res.tl, truth(one) - 0.5, atol=1e-6)
+
+
+class NaNMember(Member):
+    """Returns NaN where the normalised log-frequency lies above 0.5."""
+
+    def forward(self, x, train=False):
+        y = super().forward(x, train)
+        f = x[..., FREQ].float().reshape(-1)
+        return torch.where(f > 0.5, torch.full_like(y, float("nan")), y)
+
+
+def test_nan_member_routes_its_queries_to_pe():
+    """A member that returns NaN makes the ensemble spread NaN on those
+    queries: they count as disagreement (routed to the PE march), never as
+    accepted surrogate answers, in the spot sample and in the rest."""
+    q = D.wedge_queries(400, 15)
+    f = global_features(DOM, q, device="cpu")[:, FREQ].numpy()
+    members = [Member(), Member(), Member(), NaNMember()]
+    r = Router(DOM, members, spot_fraction=0.05, pe=make_pe(0.0), device="cpu", batch=64)
+    res = r.run(q)
+    hi = f > 0.5 + 0.01
+    lo = f < 0.5 - 0.01
+    assert hi.sum() > 50 and lo.sum() > 50
+    assert res.routed[hi].all() and not res.routed[lo].any()
+    assert not res.batch_fallback
+    assert torch.isfinite(res.tl).all()
+    t = truth(q)
+    assert torch.allclose(res.tl[torch.as_tensor(np.nonzero(hi)[0])], t[torch.as_tensor(np.nonzero(hi)[0])])
+    assert len(np.intersect1d(res.spot, np.nonzero(hi)[0])) > 0          # the spot sample saw NaN answers too
+
+
+def test_nan_spot_error_falls_back():
+    """Every member NaN: the spot check cannot judge the ensemble, so no
+    surrogate answer survives (all queries answered by the PE)."""
+    q = D.wedge_queries(200, 16)
+
+    class AllNaN(Member):
+        def forward(self, x, train=False):
+            return torch.full_like(super().forward(x, train), float("nan"))
+
+    r = Router(DOM, [AllNaN() for _ in range(4)], pe=make_pe(0.0), device="cpu", batch=64)
+    res = r.run(q)
+    assert res.routed.all()
+    assert torch.isfinite(res.tl).all()
+    assert torch.allclose(res.tl, truth(q))
