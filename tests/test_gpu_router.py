@@ -3,7 +3,7 @@ ensembles and the real complex64 PE march (no stand-ins):
 
   * wedge (upslope 200 -> 20 m, sediment over basement, 50-1000 Hz): the
     1 % spot check fails, the whole batch falls back, and every answer is
-    bitwise the complex64 march's answer for that query;
+    the complex64 march's answer for that query;
   * deep water (Munk, 80-125 Hz): the ensemble answers, the routed fraction
     and the end-to-end TL error against the complex128 march are within the
     3 dB gate;
@@ -16,6 +16,21 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
+
+
+# The router marches its routed queries in batches of its own (the spot
+# sample, then the rest), and a test re-marches them in one batch.  The answers
+# agree to well below the surrogate's error (3-6 dB) but not bitwise: cell
+# means are accumulated with FP32 atomic adds from the lanes that share a depth
+# bin, and a batch marched on one shared grid takes its absorbing-layer width
+# and depth extent from the batch (lambda = mean c0 / highest frequency,
+# cupyram.py:481-498), so a complex64 field rounds differently from batch to
+# batch.
+MARCH_AGREEMENT_DB = 0.05
+
+
+def _max_db(dom, T, a, b):
+    return float(torch.max(torch.abs(T.clipped(dom, a) - T.clipped(dom, b))))
 
 
 @pytest.fixture(scope="module")
@@ -37,7 +52,7 @@ def test_wedge_router_falls_back_to_the_march(require_gpu, surrogate):
     assert len(res.spot) == 20
     assert res.batch_fallback and res.routed.all()
     pe, _, _ = T.pe_fields(dom, q, "c64", log=None)
-    assert torch.equal(res.tl, pe)
+    assert _max_db(dom, T, res.tl, pe) <= MARCH_AGREEMENT_DB
 
 
 def test_deep_router_answers_within_gate(require_gpu, surrogate):
@@ -56,7 +71,7 @@ def test_deep_router_answers_within_gate(require_gpu, surrogate):
     routed = np.nonzero(res.routed)[0]
     if len(routed):
         pe, _, _ = T.pe_fields(dom, q.take(routed), "c64", log=None)
-        assert torch.equal(res.tl[torch.as_tensor(routed, device="cuda")], pe)
+        assert _max_db(dom, T, res.tl[torch.as_tensor(routed, device="cuda")], pe) <= MARCH_AGREEMENT_DB
 
 
 def test_trained_network_beats_no_network_baselines(require_gpu, surrogate):
