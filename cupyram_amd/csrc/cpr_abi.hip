@@ -22,6 +22,7 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
 bool team_shape_supported(int precision, int rows, int lanes);
 void team_auto_shape(int precision, int nz, int *rows, int *lanes);
 int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays);
+int64_t team_mbuf_bytes(int rows, int lanes, int nrays);
 
 // Ray-major gather of a depth-major output array, for rays b0 .. b0+nb-1:
 // source element (t, i, b) at (t * nz + i) * ld + b, destination element
@@ -80,12 +81,19 @@ struct cpr_plan {
   float *cells_d = nullptr;
   double *tlcell_d = nullptr;
   int32_t *team_rays_d = nullptr;
-  char *dbuf_d = nullptr;
+  char *dbuf_d = nullptr;     // team coefficients (all team shapes)
+  double *mbuf_d = nullptr;   // team media cache
   double *stage_d = nullptr;  // staging buffer of the ray-major output reads
   int64_t stage_n = 0;        // its size in doubles
   int variant = CPR_VARIANT_THREAD;
   int team_rows = 0, team_lanes = 0;
-  std::vector<hipEvent_t> events;  // pairs around march launches
+  // hipEvent pairs around march launches not yet read, the time of those
+  // already read, and completed events kept for reuse (a caller that marches
+  // one step per call records two events per call)
+  std::vector<hipEvent_t> events;
+  std::vector<hipEvent_t> free_events;
+  double done_ms = 0.0;
+  int done_launches = 0;
   int64_t bytes = 0;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -119,6 +127,9 @@ struct cpr_plan {
     a.team_rays = team_rays_d;
     a.team_nrays = nrays;
     a.dbuf = dbuf_d;
+    a.mbuf = mbuf_d;
+    a.team_rows = variant == CPR_VARIANT_TEAM ? team_rows : 0;
+    a.team_lanes = variant == CPR_VARIANT_TEAM ? team_lanes : 0;
     return a;
   }
   size_t real_size() const { return cfg.precision == CPR_C64 ? sizeof(float) : sizeof(double); }
@@ -135,25 +146,32 @@ int dalloc(cpr_plan *p, T **ptr, size_t count) {
   return 0;
 }
 
+// Elements of column k: depth-major [nz_max+2][ld]; the field of a team plan
+// ([ray][k][lane]) may need more (rows x lanes can exceed nz_max + 2).
+size_t column_cells(const cpr_plan *p, int k) {
+  size_t cells = static_cast<size_t>(p->nz_max + 2) * static_cast<size_t>(p->ld);
+  if (k == 4 && p->variant == CPR_VARIANT_TEAM)
+    cells = std::max(cells, static_cast<size_t>(p->nrays) * p->team_rows * p->team_lanes);
+  return cells;
+}
+
 int alloc_columns(cpr_plan *p, void **col, size_t rs, bool only_u = false) {
-  const size_t cells = static_cast<size_t>(p->nz_max + 2) * static_cast<size_t>(p->ld);
   const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
   for (int k = 0; k < 7; ++k) {
     if (only_u && k != 4) continue;
     char *q = nullptr;
-    if (dalloc(p, &q, cells * sizes[k])) return -1;
+    if (dalloc(p, &q, column_cells(p, k) * sizes[k])) return -1;
     col[k] = q;
   }
   return 0;
 }
 
 void free_columns(cpr_plan *p, void **col, size_t rs) {
-  const size_t cells = static_cast<size_t>(p->nz_max + 2) * static_cast<size_t>(p->ld);
   const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
   for (int k = 0; k < 7; ++k) {
     if (col[k]) {
       (void)hipFree(col[k]);
-      p->bytes -= static_cast<int64_t>(std::max<size_t>(cells * sizes[k], 1));
+      p->bytes -= static_cast<int64_t>(std::max<size_t>(column_cells(p, k) * sizes[k], 1));
       col[k] = nullptr;
     }
   }
@@ -191,6 +209,36 @@ int check_ray(const cpr_ray &r, const cpr_env &e, int64_t env_len, int b) {
   return 0;
 }
 
+// An event for a march launch: a completed one from the pool, or a new one.
+int take_event(cpr_plan *p, hipEvent_t *e) {
+  if (!p->free_events.empty()) {
+    *e = p->free_events.back();
+    p->free_events.pop_back();
+    return 0;
+  }
+  HIPCHK(hipEventCreate(e));
+  return 0;
+}
+
+// Adds the time of every completed launch (in launch order) to the plan's
+// total and returns its two events to the pool.
+int fold_events(cpr_plan *p) {
+  size_t i = 0;
+  for (; i + 1 < p->events.size(); i += 2) {
+    const hipError_t q = hipEventQuery(p->events[i + 1]);
+    if (q == hipErrorNotReady) break;
+    if (q != hipSuccess) return fail(std::string("hipEventQuery: ") + hipGetErrorString(q));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, p->events[i], p->events[i + 1]));
+    p->done_ms += t;
+    p->done_launches += 1;
+    p->free_events.push_back(p->events[i]);
+    p->free_events.push_back(p->events[i + 1]);
+  }
+  p->events.erase(p->events.begin(), p->events.begin() + static_cast<std::ptrdiff_t>(i));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -212,7 +260,8 @@ int cpr_device_info(int device, char *name, int name_len, int64_t *free_bytes, i
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
   if (name && name_len > 0) {
-    std::snprintf(name, static_cast<size_t>(name_len), "%s", prop.name);
+    // the marketing name, or the architecture where the runtime leaves it empty
+    std::snprintf(name, static_cast<size_t>(name_len), "%s", prop.name[0] ? prop.name : prop.gcnArchName);
   }
   if (free_bytes || total_bytes) {
     int cur = 0;
@@ -249,12 +298,13 @@ int cpr_plan_destroy(cpr_plan *p) {
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   for (auto e : p->events) (void)hipEventDestroy(e);
+  for (auto e : p->free_events) (void)hipEventDestroy(e);
   free_columns(p, p->col, p->real_size());
   for (void *q : {static_cast<void *>(p->rays_d), static_cast<void *>(p->envs_d), static_cast<void *>(p->env_d),
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
                   static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d),
                   static_cast<void *>(p->team_rays_d), static_cast<void *>(p->dbuf_d),
-                  static_cast<void *>(p->stage_d),
+                  static_cast<void *>(p->mbuf_d), static_cast<void *>(p->stage_d),
                   static_cast<void *>(p->cells_d),
                   static_cast<void *>(p->tlcell_d)})
     if (q) (void)hipFree(q);
@@ -333,7 +383,9 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     for (int b = 0; b < cfg->nrays; ++b) ids[b] = b;
     if (dalloc(p, &p->team_rays_d, cfg->nrays)) return bail(-1);
     const int64_t db = team_dbuf_bytes(cfg->precision, p->team_rows, p->team_lanes, cfg->nrays);
-    if (db > 0 && dalloc(p, &p->dbuf_d, static_cast<size_t>(db))) return bail(-1);
+    if (dalloc(p, &p->dbuf_d, static_cast<size_t>(db))) return bail(-1);
+    const int64_t mb = team_mbuf_bytes(p->team_rows, p->team_lanes, cfg->nrays);
+    if (dalloc(p, &p->mbuf_d, static_cast<size_t>(mb / sizeof(double)))) return bail(-1);
     // on the plan's stream, behind dalloc's clearing of the same buffer: the
     // stream is non-blocking, so a null-stream copy could land before it
     if (hipMemcpyAsync(p->team_rays_d, ids.data(), sizeof(int32_t) * cfg->nrays, hipMemcpyHostToDevice,
@@ -363,8 +415,11 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
 
 int cpr_selfs(cpr_plan *p) {
   if (!p) return fail("null plan");
-  for (auto e : p->events) (void)hipEventDestroy(e);
+  if (fold_events(p)) return -1;
+  for (auto e : p->events) (void)hipEventDestroy(e);  // launches of an earlier march still in flight
   p->events.clear();
+  p->done_ms = 0.0;
+  p->done_launches = 0;
   MarchArgs a = p->args();
   void *dcol[7] = {};
   const bool own = !(p->cfg.precision == CPR_C128 && p->variant == CPR_VARIANT_THREAD);
@@ -425,8 +480,7 @@ int cpr_march(cpr_plan *p, int step_begin, int step_end) {
   step_end = std::min(step_end, p->max_steps);
   if (step_end <= step_begin) return 0;
   hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  if (fold_events(p) || take_event(p, &e0) || take_event(p, &e1)) return -1;
   p->events.push_back(e0);
   p->events.push_back(e1);
   HIPCHK(hipEventRecord(e0, p->stream));
@@ -460,8 +514,7 @@ int cpr_march_graph(cpr_plan *p, int steps_per_launch) {
   p->graph = g;
   HIPCHK(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
   hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  if (fold_events(p) || take_event(p, &e0) || take_event(p, &e1)) return -1;
   p->events.push_back(e0);
   p->events.push_back(e1);
   HIPCHK(hipEventRecord(e0, p->stream));
@@ -479,14 +532,9 @@ int cpr_sync(cpr_plan *p) {
 int cpr_march_time(cpr_plan *p, double *ms, int *nlaunch) {
   if (!p) return fail("null plan");
   HIPCHK(hipStreamSynchronize(p->stream));
-  double tot = 0;
-  for (size_t i = 0; i + 1 < p->events.size(); i += 2) {
-    float t = 0;
-    HIPCHK(hipEventElapsedTime(&t, p->events[i], p->events[i + 1]));
-    tot += t;
-  }
-  if (ms) *ms = tot;
-  if (nlaunch) *nlaunch = static_cast<int>(p->events.size() / 2);
+  if (fold_events(p)) return -1;
+  if (ms) *ms = p->done_ms;
+  if (nlaunch) *nlaunch = p->done_launches;
   return 0;
 }
 
@@ -569,22 +617,30 @@ int cpr_read_field(cpr_plan *p, double *u, int nz_max) {
   HIPCHK(hipStreamSynchronize(p->stream));
   const int64_t ld = p->ld, rows = p->nz_max + 2, n = p->nrays;
   const bool c64 = p->cfg.precision == CPR_C64;
-  const size_t bytes = static_cast<size_t>(rows * ld) * (c64 ? 8 : 16);
+  const bool team = p->variant == CPR_VARIANT_TEAM;
+  const int64_t M = p->team_rows, T = p->team_lanes;
+  const size_t bytes = column_cells(p, 4) * (c64 ? 8 : 16);
   std::vector<char> buf(bytes);
   HIPCHK(hipMemcpyAsync(buf.data(), p->col[4], bytes, hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
   std::memset(u, 0, sizeof(double) * 2 * n * (nz_max + 2));
   for (int64_t b = 0; b < n; ++b)
     for (int64_t i = 0; i < rows; ++i) {
+      // team plans keep rows 1..nz as [ray][k][lane]; rows 0 and nz+1 are zero
+      int64_t e = i * ld + b;
+      if (team) {
+        if (i < 1 || i > p->rays_h[b].nz) continue;
+        e = (b * M + (i - 1) % M) * T + (i - 1) / M;
+      }
       double re, im;
       if (c64) {
         const float *f = reinterpret_cast<const float *>(buf.data());
-        re = f[2 * (i * ld + b)];
-        im = f[2 * (i * ld + b) + 1];
+        re = f[2 * e];
+        im = f[2 * e + 1];
       } else {
         const double *d = reinterpret_cast<const double *>(buf.data());
-        re = d[2 * (i * ld + b)];
-        im = d[2 * (i * ld + b) + 1];
+        re = d[2 * e];
+        im = d[2 * e + 1];
       }
       u[2 * (b * (nz_max + 2) + i)] = re;
       u[2 * (b * (nz_max + 2) + i) + 1] = im;

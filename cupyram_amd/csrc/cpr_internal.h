@@ -57,5 +57,10 @@ struct MarchArgs {
   // team variant
   const int32_t *team_rays;  // rays handled by this launch (team variant)
   int team_nrays;
-  void *dbuf;                // streamed Galerkin coefficients (deep teams)
+  void *dbuf;                // Galerkin coefficients of every team row (streamed
+                             // by deep teams; saved between launches by the others)
+  void *mbuf;                // FP64 water and bottom media of every team row
+  // layout of the field u: team_rows = 0: depth-major [row][ld]; > 0: the
+  // team layout [ray][k][lane] of team_rows rows x team_lanes lanes
+  int team_rows, team_lanes;
 };

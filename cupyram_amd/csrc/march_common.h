@@ -61,17 +61,33 @@ CPR_HD inline R cabs2(cx<R> a) {
 
 // numpy.interp-style clamped linear interpolation: the last knot j with
 // x[j] <= z (a duplicated knot resolves to the right segment).
-CPR_HD inline double interp_clamped(const double *x, const double *y, int n, int ystride, double z) {
+//
+// `cursor` (optional) carries the segment from one call to the next for
+// nondecreasing z (the rows of a column, top down): the segment is then found
+// by walking forward from the previous one instead of a binary search, so a
+// column's rows cost one load per knot passed instead of a chain of
+// log2(n) dependent loads per row.  Both ways find the same segment, so the
+// results are identical.  *cursor < 0 means "not known yet".
+CPR_HD inline double interp_clamped(const double *x, const double *y, int n, int ystride, double z,
+                                    int *cursor = nullptr) {
   if (z <= x[0]) return y[0];
   if (z >= x[n - 1]) return y[(n - 1) * ystride];
-  int lo = 0, hi = n - 1;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (x[mid] <= z)
-      lo = mid;
-    else
-      hi = mid;
+  int lo;
+  if (cursor && *cursor >= 0) {
+    lo = *cursor;  // x[lo] <= (an earlier z) <= z
+    while (lo + 1 < n - 1 && x[lo + 1] <= z) ++lo;
+  } else {
+    lo = 0;
+    int hi = n - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (x[mid] <= z)
+        lo = mid;
+      else
+        hi = mid;
+    }
   }
+  if (cursor) *cursor = lo;
   const double x0 = x[lo], x1 = x[lo + 1];
   const double y0 = y[lo * ystride], y1 = y[(lo + 1) * ystride];
   if (x1 == x0) return y0;
@@ -90,8 +106,8 @@ CPR_HD inline double grid_depth(const cpr_ray &P, int row) {
   return row == P.nz + 1 ? P.zmax : static_cast<double>(row) * P.zstep;
 }
 
-CPR_HD inline double water_speed(const cpr_env &E, const double *env, int ss, double z) {
-  return interp_clamped(env + E.z_ss, env + E.cw + ss, E.n_ss, E.n_rp_ss, z);
+CPR_HD inline double water_speed(const cpr_env &E, const double *env, int ss, double z, int *cursor = nullptr) {
+  return interp_clamped(env + E.z_ss, env + E.cw + ss, E.n_ss, E.n_rp_ss, z, cursor);
 }
 
 CPR_HD inline double alpw_at(const cpr_ray &P, const cpr_env &E, const double *env, int ss, double z) {
@@ -99,11 +115,13 @@ CPR_HD inline double alpw_at(const cpr_ray &P, const cpr_env &E, const double *e
 }
 
 // Water medium of one grid row (FP64): f1 = 1/alpw, f2 = 1, f3 = alpw,
-// ksq = (omega/cw)^2 - k0^2.
-CPR_HD inline MediumRow water_medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int ss, int row) {
+// ksq = (omega/cw)^2 - k0^2.  `cursor`: see interp_clamped (rows in
+// increasing order).
+CPR_HD inline MediumRow water_medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int ss, int row,
+                                        int *cursor = nullptr) {
   const double z = grid_depth(P, row);
   MediumRow m;
-  const double cw = water_speed(E, env, ss, z);
+  const double cw = water_speed(E, env, ss, z, cursor);
   const double a = sqrt(cw / P.c0);
   m.f1 = 1.0 / a;
   m.f2 = 1.0;
@@ -117,21 +135,22 @@ CPR_HD inline MediumRow water_medium_at(const cpr_ray &P, const cpr_env &E, cons
 // Bottom medium of one grid row (FP64): f1 = rhob/alpb, f2 = 1/rhob,
 // f3 = alpb, ksq = ((omega/cb)(1 + i eta attn))^2 - k0^2 with the absorbing
 // layer ramp below the last seabed depth.
-CPR_HD inline MediumRow bottom_medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int sb, int row) {
+CPR_HD inline MediumRow bottom_medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int sb, int row,
+                                         int *cursor = nullptr) {
   const double z = grid_depth(P, row);
   MediumRow m;
   const double k0sq = P.k0 * P.k0;
   {
     const double *zsb = env + E.z_sb;
     const int nsb = E.n_sb, st = E.n_rp_sb;
-    const double cb = interp_clamped(zsb, env + E.cb + sb, nsb, st, z);
-    const double rh = interp_clamped(zsb, env + E.rhob + sb, nsb, st, z);
+    const double cb = interp_clamped(zsb, env + E.cb + sb, nsb, st, z, cursor);
+    const double rh = interp_clamped(zsb, env + E.rhob + sb, nsb, st, z, cursor);
     const double zl = zsb[nsb - 1];
     const double at_last = env[E.attn + sb + (nsb - 1) * st];
     const double z1 = zl + 0.75 * P.lyrw_lambda, z2 = zl + P.lyrw_lambda;
     double at;
     if (z <= zl)
-      at = interp_clamped(zsb, env + E.attn + sb, nsb, st, z);
+      at = interp_clamped(zsb, env + E.attn + sb, nsb, st, z, cursor);
     else if (z <= z1)
       at = at_last;
     else if (z <= z2)
@@ -152,8 +171,8 @@ CPR_HD inline MediumRow bottom_medium_at(const cpr_ray &P, const cpr_env &E, con
 
 // Medium of one grid row for water/bottom selection by iz (FP64).
 CPR_HD inline MediumRow medium_at(const cpr_ray &P, const cpr_env &E, const double *env, int ss, int sb, int iz,
-                                  int row) {
-  return row <= iz ? water_medium_at(P, E, env, ss, row) : bottom_medium_at(P, E, env, sb, row);
+                                  int row, int *wcur = nullptr, int *bcur = nullptr) {
+  return row <= iz ? water_medium_at(P, E, env, ss, row, wcur) : bottom_medium_at(P, E, env, sb, row, bcur);
 }
 
 // Range update of one ray before a march step at range r.  Returns a bit

@@ -312,39 +312,48 @@ struct Coefs<R, M, true> {
 };
 
 // Water and bottom media of the lane's rows g0-1 .. g0+M (FP64), kept in a
-// per-lane private array.  Both sides are evaluated when the profiles change
-// (once per launch and at profile range switches); when only the seabed row
-// iz moves, the coefficients of the rows it touches are re-formed from this
-// cache without interpolating the profiles again.
+// per-plan device buffer that persists between launches, laid out
+// [ray][q][field][lane] (q = 0 .. M+1 for rows g0-1+q; fields 0-4 the water
+// f1 f2 f3 kre kim, 5-9 the bottom ones) so that a team's accesses to one row
+// are coalesced.  Both sides are evaluated after the self-starter and at
+// profile range switches; when only the seabed row iz moves, the
+// coefficients of the rows it touches are re-formed from it without
+// interpolating the profiles again, and the TL output reads f3 from it.  (A
+// per-lane private copy, filled at every launch, cost about 2.6 ms of scratch
+// traffic per launch of the 10,000-ray Munk batch.)
 template <int M>
 struct MedCache {
-  double w[M + 2][5];
-  double b[M + 2][5];
+  double *p;  // this lane's element of (q = 0, field 0)
+  int T;      // lanes per team: the stride between fields
+  __device__ double get(int q, int f) const { return p[static_cast<int64_t>(q * 10 + f) * T]; }
+  __device__ void set(int q, int f, double v) const { p[static_cast<int64_t>(q * 10 + f) * T] = v; }
 };
 
-__device__ inline void put_med(double (&d)[5], const MediumRow &m) {
-  d[0] = m.f1;
-  d[1] = m.f2;
-  d[2] = m.f3;
-  d[3] = m.kre;
-  d[4] = m.kim;
+template <int M>
+__device__ inline void put_med(const MedCache<M> &mc, int q, int o, const MediumRow &m) {
+  mc.set(q, o + 0, m.f1);
+  mc.set(q, o + 1, m.f2);
+  mc.set(q, o + 2, m.f3);
+  mc.set(q, o + 3, m.kre);
+  mc.set(q, o + 4, m.kim);
 }
 
 template <int M>
-__device__ void fill_media(MedCache<M> &mc, const cpr_ray &P, const cpr_env &E, const double *env,
+__device__ void fill_media(const MedCache<M> &mc, const cpr_ray &P, const cpr_env &E, const double *env,
                            const RayState &st, int g0) {
+  int wcur = -1, bcur = -1;  // profile segments carried down the lane's rows
 #pragma unroll 1
   for (int q = 0; q < M + 2; ++q) {
     const int row = g0 - 1 + q;
     if (row <= P.nz + 1) {
-      put_med(mc.w[q], water_medium_at(P, E, env, st.ss_ind, row));
-      put_med(mc.b[q], bottom_medium_at(P, E, env, st.sb_ind, row));
+      put_med(mc, q, 0, water_medium_at(P, E, env, st.ss_ind, row, &wcur));
+      put_med(mc, q, 5, bottom_medium_at(P, E, env, st.sb_ind, row, &bcur));
     } else {
       MediumRow m;
       m.f1 = m.f2 = m.f3 = 1.0;
       m.kre = m.kim = 0.0;
-      put_med(mc.w[q], m);
-      put_med(mc.b[q], m);
+      put_med(mc, q, 0, m);
+      put_med(mc, q, 5, m);
     }
   }
 }
@@ -355,9 +364,14 @@ __device__ void fill_media(MedCache<M> &mc, const cpr_ray &P, const cpr_env &E, 
 template <class R, int M>
 __device__ CoefRow<R> coef_from_cache(const MedCache<M> &mc, int k, int g0, int iz, double cfact) {
   const double dfact = 1.0 / 12.0;
-  const double *m0 = (g0 + k - 1 <= iz) ? mc.w[k] : mc.b[k];
-  const double *m1 = (g0 + k <= iz) ? mc.w[k + 1] : mc.b[k + 1];
-  const double *m2 = (g0 + k + 1 <= iz) ? mc.w[k + 2] : mc.b[k + 2];
+  const int o0 = (g0 + k - 1 <= iz) ? 0 : 5;
+  const int o1 = (g0 + k <= iz) ? 0 : 5;
+  const int o2 = (g0 + k + 1 <= iz) ? 0 : 5;
+  const double m0[5] = {0.0, mc.get(k, o0 + 1), mc.get(k, o0 + 2), mc.get(k, o0 + 3), mc.get(k, o0 + 4)};
+  const double m1[5] = {mc.get(k + 1, o1), mc.get(k + 1, o1 + 1), mc.get(k + 1, o1 + 2), mc.get(k + 1, o1 + 3),
+                        mc.get(k + 1, o1 + 4)};
+  const double m2[5] = {0.0, mc.get(k + 2, o2 + 1), mc.get(k + 2, o2 + 2), mc.get(k + 2, o2 + 3),
+                        mc.get(k + 2, o2 + 4)};
   const double c1 = cfact * m1[0] * (m0[1] + m1[1]) * m0[2];
   const double c2 = -cfact * m1[0] * (m0[1] + 2.0 * m1[1] + m2[1]) * m1[2];
   const double c3 = cfact * m1[0] * (m1[1] + m2[1]) * m2[2];
@@ -372,7 +386,7 @@ __device__ CoefRow<R> coef_from_cache(const MedCache<M> &mc, int k, int g0, int 
 // f3 (alpw or alpb) of lane row k from the cached media, for the TL output.
 template <int M>
 __device__ inline double f3_from_cache(const MedCache<M> &mc, int k, int g0, int iz) {
-  return (g0 + k <= iz) ? mc.w[k + 1][2] : mc.b[k + 1][2];
+  return mc.get(k + 1, ((g0 + k <= iz) ? 0 : 5) + 2);
 }
 
 // Cell-mean intensity: adds |f3 u|^2 * rinv of the lane's rows to the output
@@ -400,6 +414,29 @@ __device__ void add_cells(const MarchArgs &a, const cpr_ray &P, int b, int bin, 
       ++j;
       left = ndz;
     }
+  }
+}
+
+// Coefficients a lane keeps in VGPRs are saved to the team buffer at the end
+// of a launch and loaded at the start of the next one, in the layout of the
+// streamed rows (real halves [ray][k][lane], then the imaginary halves), so a
+// launch does not rebuild them from the media.
+template <class R, int M>
+__device__ void save_coefs(const Coefs<R, M, false> &C, CoefHalf<R> *re, CoefHalf<R> *im, int T, int t) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    (re + k * T)[t] = CoefHalf<R>{C.r[k].d1.re, C.r[k].d2.re, C.r[k].d3.re};
+    (im + k * T)[t] = CoefHalf<R>{C.r[k].d1.im, C.r[k].d2.im, C.r[k].d3.im};
+  }
+}
+template <class R, int M>
+__device__ void load_coefs(Coefs<R, M, false> &C, const CoefHalf<R> *re, const CoefHalf<R> *im, int T, int t) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const CoefHalf<R> a = (re + k * T)[t], b = (im + k * T)[t];
+    C.r[k].d1 = cxm<R>(a.d1, b.d1);
+    C.r[k].d2 = cxm<R>(a.d2, b.d2);
+    C.r[k].d3 = cxm<R>(a.d3, b.d3);
   }
 }
 
@@ -512,23 +549,33 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   LaneCol<R, M> L;
   Coefs<R, M, DS> C;
   Elim<R, M> X;
-  MedCache<M> mc;
   const double cfact = 0.5 / (P.dz * P.dz);
+  CoefHalf<R> *const cre = static_cast<CoefHalf<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T;
+  CoefHalf<R> *const cim = static_cast<CoefHalf<R> *>(a.dbuf) + (static_cast<int64_t>(a.nrays) + b) * M * T;
   if constexpr (DS) {
-    C.re = static_cast<CoefHalf<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T;
-    C.im = static_cast<CoefHalf<R> *>(a.dbuf) + (static_cast<int64_t>(a.nrays) + b) * M * T;
+    C.re = cre;
+    C.im = cim;
     C.T = T;
     C.t = t;
   }
-  cx<R> *ug = static_cast<cx<R> *>(a.u);
+  const MedCache<M> mc{static_cast<double *>(a.mbuf) + static_cast<int64_t>(b) * (M + 2) * 10 * T + t, T};
+  // the field in the team layout [ray][k][lane] (rows 1 + t M + k): a team's
+  // loads and stores of one k are contiguous
+  cx<R> *const ug = static_cast<cx<R> *>(a.u) + static_cast<int64_t>(b) * M * T + t;
 #pragma unroll
-  for (int k = 0; k < M; ++k) L.u[k] = k < mv ? ug[static_cast<int64_t>(g0 + k) * a.ld + b] : cxm<R>(0, 0);
+  for (int k = 0; k < M; ++k) L.u[k] = k < mv ? ug[static_cast<int64_t>(k) * T] : cxm<R>(0, 0);
   // receiver rows: owner lane / slot of rows ir and ir+1
   const double ri = P.zr / P.dz;
   const int ir = static_cast<int>(floor(ri));
   const double dir = ri - ir;
-  bool fresh = true;
   const int end = s1 < P.nsteps ? s1 : P.nsteps;
+  // The media and the coefficients persist between launches: they are
+  // rebuilt in full only after the self-starter (dirty) and at profile
+  // switches; VGPR-resident coefficients come back from the team buffer.
+  bool full = st.dirty != 0;
+  if constexpr (!DS) {
+    if (!full && s0 < end) load_coefs<R, M>(C, cre, cim, T, t);
+  }
   for (int step = s0; step < end; ++step) {
     const double r = (step + 1) * P.dr;
     const int jz = st.iz;
@@ -536,13 +583,14 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
     st = make_uniform(st);
     // f changes on rows (lo, hi] when iz moves; d changes on rows lo .. hi+1
     const int lo = jz < st.iz ? jz : st.iz, hi = jz > st.iz ? jz : st.iz;
-    if (fresh || (fl & 2)) {
+    if (full || (fl & 2)) {
       fill_media<M>(mc, P, E, a.env, st, g0);
       rebuild_rows<R, M, DS>(C, mc, g0, st.iz, cfact, 0, M - 1);
+      full = false;
+      st.dirty = 0;
     } else if ((fl & 1) && g0 <= hi + 1 && g0 + M - 1 >= lo) {
       rebuild_rows<R, M, DS>(C, mc, g0, st.iz, cfact, lo - g0, hi + 1 - g0);
     }
-    fresh = false;
     // every row stencil of this wavefront above the seabed row: real coefficients
     const bool wreal = __builtin_amdgcn_ballot_w64(g0 + M > st.iz) == 0;
     const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
@@ -682,7 +730,10 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   }
 #pragma unroll
   for (int k = 0; k < M; ++k)
-    if (k < mv) ug[static_cast<int64_t>(g0 + k) * a.ld + b] = L.u[k];
+    if (k < mv) ug[static_cast<int64_t>(k) * T] = L.u[k];
+  if constexpr (!DS) {
+    if (s0 < end) save_coefs<R, M>(C, cre, cim, T, t);
+  }
   if (t == 0) a.state[b] = st;
 }
 
@@ -782,8 +833,11 @@ void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
   *lanes = best_l;
 }
 
+int64_t team_mbuf_bytes(int rows, int lanes, int nrays) {
+  return static_cast<int64_t>(nrays) * (rows + 2) * 10 * lanes * static_cast<int64_t>(sizeof(double));
+}
+
 int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays) {
-  if (!team_rows_stream(precision, rows)) return 0;
   const int64_t row_bytes = precision == CPR_C64 ? sizeof(CoefRow<float>) : sizeof(CoefRow<double>);
   return static_cast<int64_t>(nrays) * rows * lanes * row_bytes;
 }

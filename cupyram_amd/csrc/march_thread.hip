@@ -51,8 +51,9 @@ __device__ Col<R> col_of(const MarchArgs &a, int b) {
 template <class R>
 __device__ void materialize(const Col<R> &c, const cpr_ray &P, const cpr_env &E, const double *env,
                             const RayState &st, int lo, int hi) {
+  int wcur = -1, bcur = -1;  // profile segments carried down the rows
   for (int i = lo; i <= hi; ++i) {
-    const MediumRow m = medium_at(P, E, env, st.ss_ind, st.sb_ind, st.iz, i);
+    const MediumRow m = medium_at(P, E, env, st.ss_ind, st.sb_ind, st.iz, i, &wcur, &bcur);
     c.F1(i) = static_cast<R>(m.f1);
     c.F2(i) = static_cast<R>(m.f2);
     c.F3(i) = static_cast<R>(m.f3);
@@ -202,9 +203,20 @@ template <class R>
 __global__ void k_field_from_starter(MarchArgs a, const cx<double> *ud) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= a.nrays) return;
-  const int rows = a.rays[b].nz + 2;
+  const int nz = a.rays[b].nz;
   cx<R> *u = static_cast<cx<R> *>(a.u);
-  for (int i = 0; i < rows; ++i) {
+  if (a.team_rows > 0) {
+    // team layout: row i = 1 + t M + k at [b][k][t]
+    const int M = a.team_rows, T = a.team_lanes;
+    cx<R> *ub = u + static_cast<int64_t>(b) * M * T;
+    for (int i = 1; i <= nz; ++i) {
+      const cx<double> v = ud[i * a.ld + b];
+      const int t = (i - 1) / M, k = (i - 1) % M;
+      ub[static_cast<int64_t>(k) * T + t] = cxm<R>(static_cast<R>(v.re), static_cast<R>(v.im));
+    }
+    return;
+  }
+  for (int i = 0; i < nz + 2; ++i) {
     const cx<double> v = ud[i * a.ld + b];
     u[i * a.ld + b] = cxm<R>(static_cast<R>(v.re), static_cast<R>(v.im));
   }

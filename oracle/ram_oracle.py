@@ -21,6 +21,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_NATIVE = {}
 
 
 class _Ray(C.Structure):
@@ -41,27 +42,40 @@ class _Ray(C.Structure):
     ]
 
 
+def _bind(L):
+    dp = C.POINTER(C.c_double)
+    L.orc_pade.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, dp, dp]
+    L.orc_dims.argtypes = [C.POINTER(_Ray)] + [C.POINTER(C.c_int)] * 4
+    L.orc_run.argtypes = [C.POINTER(_Ray), dp, dp, dp, dp, C.POINTER(C.c_int)]
+    L.orc_run_trace.argtypes = [C.POINTER(_Ray), dp, C.POINTER(C.c_int), dp]
+    L.orc_run_batch.argtypes = [C.POINTER(_Ray), C.c_int, C.POINTER(dp), C.c_int]
+    L.orc_run_cells.argtypes = [C.POINTER(_Ray), dp, dp]
+    L.orc_profiles.argtypes = [C.POINTER(_Ray), C.c_int, C.c_int, dp, dp, dp, dp, dp]
+    L.orc_apply_factors.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, dp,
+                                    dp, dp, dp, dp, dp, dp]
+    return L
+
+
 def lib():
-    """Load (building on first use if needed) liboracle."""
+    """Load (building on first use if needed) liboracle: the checker build."""
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "libram_oracle.so")
         if not os.path.exists(path):
             import subprocess
             subprocess.check_call(["make", "-s", "-C", _HERE])
-        L = C.CDLL(path)
-        dp = C.POINTER(C.c_double)
-        L.orc_pade.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, dp, dp]
-        L.orc_dims.argtypes = [C.POINTER(_Ray)] + [C.POINTER(C.c_int)] * 4
-        L.orc_run.argtypes = [C.POINTER(_Ray), dp, dp, dp, dp, C.POINTER(C.c_int)]
-        L.orc_run_trace.argtypes = [C.POINTER(_Ray), dp, C.POINTER(C.c_int), dp]
-        L.orc_run_batch.argtypes = [C.POINTER(_Ray), C.c_int, C.POINTER(dp), C.c_int]
-        L.orc_run_cells.argtypes = [C.POINTER(_Ray), dp, dp]
-        L.orc_profiles.argtypes = [C.POINTER(_Ray), C.c_int, C.c_int, dp, dp, dp, dp, dp]
-        L.orc_apply_factors.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp, dp,
-                                        dp, dp, dp, dp, dp, dp]
-        _LIB = L
+        _LIB = _bind(C.CDLL(path))
     return _LIB
+
+
+def native_lib(directory):
+    """The same source built with -O3 -march=native into `directory` on this
+    machine (the timed CPU baseline); built on first use."""
+    if directory not in _NATIVE:
+        import subprocess
+        subprocess.check_call(["make", "-s", "-C", _HERE, "native", f"NATIVE_DIR={directory}"])
+        _NATIVE[directory] = _bind(C.CDLL(os.path.join(directory, "libram_oracle_native.so")))
+    return _NATIVE[directory]
 
 
 def _dp(a):
@@ -209,8 +223,9 @@ def run_cells(ray: OracleRay):
     return {"TL Line": tll, "TL Cells": tlc}
 
 
-def run_batch_tl(rays, nthreads=None):
-    """TL lines for many rays on OpenMP threads (the CPU baseline)."""
+def run_batch_tl(rays, nthreads=None, library=None):
+    """TL lines for many rays on OpenMP threads (the CPU baseline); `library`
+    is lib() (default) or native_lib(dir)."""
     structs = (_Ray * len(rays))(*[r.struct() for r in rays])
     outs = []
     for r in rays:
@@ -218,7 +233,7 @@ def run_batch_tl(rays, nthreads=None):
         outs.append(np.zeros(nvr))
     ptrs = (C.POINTER(C.c_double) * len(rays))(*[_dp(o) for o in outs])
     nthreads = nthreads or os.cpu_count() or 1
-    bad = lib().orc_run_batch(structs, len(rays), ptrs, int(nthreads))
+    bad = (library or lib()).orc_run_batch(structs, len(rays), ptrs, int(nthreads))
     if bad:
         raise ValueError(f"{bad} oracle rays failed")
     return outs

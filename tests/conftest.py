@@ -21,8 +21,22 @@ def gpu_available():
         return False
 
 
+def _start_torch_first():
+    """torch ships its own HIP runtime under the soname of ROCm's
+    (libamdhip64.so.7).  Whichever loads first serves the whole process, and
+    torch cannot start on ROCm's copy; so torch's goes first, and tests that
+    use torch (device buffers, the surrogate) run in any order."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.zeros(1, device="cuda")
+
+
 @pytest.fixture(scope="session")
 def require_gpu():
+    _start_torch_first()
     from cupyram_amd import _abi
     n = _abi.device_count()
     if n < 1:

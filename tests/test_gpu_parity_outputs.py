@@ -163,3 +163,32 @@ def test_device_info_reports_the_device(require_gpu):
     name, free, total = _abi.device_info(0)
     print("device", name, free / 2**30, total / 2**30)
     assert name and 0 < free <= total
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-9), ("c64", 2e-3)])
+def test_field_read_back_in_both_layouts(require_gpu, precision, tol):
+    """cpr_read_field returns the march field u [ray][row] from the thread
+    plan's depth-major columns and from the team plan's [ray][k][lane]
+    layout alike: the two marches of the wedge agree to rounding (complex128)
+    or within complex64's error, row by row, at every ray of the batch."""
+    kw = dict(envs.varying_ranges(), compute_grids=False)
+    fields = {}
+    for variant in ("thread", "team"):
+        m = CuPyRAM(**kw, precision=precision, variant=variant)
+        m.setup()
+        idx = np.arange(m._total_batch)
+        plan = m._make_plan(idx, variant=variant)
+        try:
+            plan.selfs()
+            plan.march(0, int(m._rays["nsteps"].max()))
+            fields[variant] = plan.read_field(int(m._rays["nz"].max()))
+        finally:
+            plan.close()
+    nz = m._rays["nz"]
+    worst = 0.0
+    for r in range(len(nz)):
+        a, b = fields["thread"][r, 1:nz[r] + 1], fields["team"][r, 1:nz[r] + 1]
+        worst = max(worst, float(np.max(np.abs(a - b)) / np.max(np.abs(a))))
+        assert np.all(fields["team"][r, nz[r] + 1:] == 0)
+    print(f"field {precision}: max relative difference thread vs team {worst:.3g}")
+    assert worst <= tol
