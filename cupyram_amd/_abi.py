@@ -86,6 +86,8 @@ _SYMBOLS = {
     "cpr_team_shape": ([C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "cpr_plan_describe": ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
                           C.c_int),
+    "cpr_march_noalloc": ([C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int,
+                           C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     # surrogate kernels (device pointers as integers)
     "cpr_fno_gemm": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                       C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
@@ -151,6 +153,21 @@ def device_info(device: int = 0):
     free, total = C.c_int64(), C.c_int64()
     check(lib().cpr_device_info(int(device), buf, len(buf), C.byref(free), C.byref(total)))
     return buf.value.decode(errors="replace"), free.value, total.value
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def march_noalloc(precision, rays, envs, env_data, pade, nvr_max, tll_dev, cpl_dev, stream=None):
+    """cpr_march_noalloc: host inputs as for Plan, device outputs (integer
+    addresses) tll [nvr_max][nrays] f64 and cpl [nvr_max][nrays] complex128."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    envs = np.ascontiguousarray(envs, ENV_DTYPE)
+    env_data = np.ascontiguousarray(env_data, np.float64)
+    pade = np.ascontiguousarray(pade, np.complex128)
+    check(lib().cpr_march_noalloc(int(precision), len(rays), ptr(rays), ptr(envs), ptr(env_data), env_data.size,
+                                  ptr(pade), int(nvr_max), C.c_void_p(tll_dev), C.c_void_p(cpl_dev),
+                                  C.c_void_p(stream) if stream else None))
 
 
 def team_shape(precision: int, nz: int):

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -669,3 +671,153 @@ int cpr_plan_describe(cpr_plan *p, int b, int *variant, int *team_rows, int *tea
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free march (cpr_march_noalloc): the thread-per-ray march of a
+// plan, with every buffer beyond the caller's inputs and outputs in one
+// file-scope device array.  In host code `scratch` names a host-side stand-in,
+// not device memory: the array's device address is looked up once per device
+// with hipGetSymbolAddress and only that address is used.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kScratchBytes = int64_t(64) << 20;  // CPR_NOALLOC_SCRATCH_BYTES
+static_assert(kScratchBytes == CPR_NOALLOC_SCRATCH_BYTES, "header and library disagree on the scratch size");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::array<unsigned char *, 64> g_scratch_ptr{};  // device address of `scratch` per device
+
+int scratch_of_current_device(unsigned char **out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= static_cast<int>(g_scratch_ptr.size())) return fail("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+__global__ void k_fill_f64(double *x, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+
+int64_t align256(int64_t n) { return (n + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" int cpr_march_noalloc(int precision, int nrays, const cpr_ray *rays, const cpr_env *envs,
+                                 const double *env_data, int64_t env_len, const double *pade, int nvr_max,
+                                 double *tll_d, double *cpl_d, void *stream) {
+  if (!rays || !envs || !env_data || !pade || !tll_d || !cpl_d) return fail("null argument");
+  if (nrays < 1) return fail("nrays must be >= 1");
+  if (precision != CPR_C64 && precision != CPR_C128) return fail("precision must be 64 or 128");
+  if (nvr_max < 0 || env_len < 1) return fail("bad sizes");
+  const int np_ = rays[0].np;
+  int nz_max = 0, max_steps = 0;
+  for (int b = 0; b < nrays; ++b) {
+    if (check_ray(rays[b], envs[b], env_len, b)) return -1;
+    if (rays[b].np != np_) return fail("all rays must use the same np");
+    if (rays[b].nvr > nvr_max) return fail("ray nvr exceeds nvr_max");
+    nz_max = std::max(nz_max, rays[b].nz);
+    max_steps = std::max(max_steps, rays[b].nsteps);
+  }
+  // scratch layout: ray tables, environment, Pade sets, march state, the
+  // march columns of the precision, and (complex64) the FP64 starter columns
+  const int64_t ld = nrays;
+  const int64_t cells = (static_cast<int64_t>(nz_max) + 2) * ld;
+  const int64_t rs = precision == CPR_C64 ? 4 : 8;
+  const int64_t col_bytes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
+  const int64_t npade = static_cast<int64_t>(nrays) * 3 * 2 * np_ * 2;
+  int64_t off = 0;
+  const int64_t o_rays = off;
+  off += align256(nrays * static_cast<int64_t>(sizeof(cpr_ray)));
+  const int64_t o_envs = off;
+  off += align256(nrays * static_cast<int64_t>(sizeof(cpr_env)));
+  const int64_t o_env = off;
+  off += align256(env_len * static_cast<int64_t>(sizeof(double)));
+  const int64_t o_pade = off;
+  off += align256(npade * static_cast<int64_t>(sizeof(double)));
+  const int64_t o_state = off;
+  off += align256(nrays * static_cast<int64_t>(sizeof(RayState)));
+  int64_t o_col[7], o_dcol[7];
+  for (int k = 0; k < 7; ++k) {
+    o_col[k] = off;
+    off += align256(cells * col_bytes[k]);
+  }
+  const bool own = precision == CPR_C64;  // complex128 starts in its own columns
+  for (int k = 0; k < 7; ++k) {
+    o_dcol[k] = off;
+    if (own) off += align256(cells * col_bytes[k] * 8 / rs);
+  }
+  if (off > kScratchBytes) {
+    char buf[200];
+    std::snprintf(buf, sizeof(buf), "batch needs %lld bytes of scratch, more than the %lld of cpr_march_noalloc",
+                  static_cast<long long>(off), static_cast<long long>(kScratchBytes));
+    return fail(buf);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
+  unsigned char *base = nullptr;
+  if (scratch_of_current_device(&base)) return -1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipMemcpyAsync(base + o_rays, rays, sizeof(cpr_ray) * nrays, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(base + o_envs, envs, sizeof(cpr_env) * nrays, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(base + o_env, env_data, sizeof(double) * env_len, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(base + o_pade, pade, sizeof(double) * npade, hipMemcpyHostToDevice, st));
+  // the outputs start at zero (entries past a ray's nvr are never written)
+  const int64_t nline = static_cast<int64_t>(std::max(nvr_max, 1)) * ld;
+  hipLaunchKernelGGL(k_fill_f64, dim3(256), dim3(256), 0, st, tll_d, nline, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(256), dim3(256), 0, st, cpl_d, 2 * nline, 0.0);
+  HIPCHK(hipGetLastError());
+  MarchArgs a{};
+  a.rays = reinterpret_cast<const cpr_ray *>(base + o_rays);
+  a.envs = reinterpret_cast<const cpr_env *>(base + o_envs);
+  a.env = reinterpret_cast<const double *>(base + o_env);
+  a.pade = reinterpret_cast<const double *>(base + o_pade);
+  a.state = reinterpret_cast<RayState *>(base + o_state);
+  a.nrays = nrays;
+  a.np_ = np_;
+  a.ld = ld;
+  a.nvr_max = nvr_max;
+  a.nvz_max = 0;
+  a.grids = 0;
+  void **cols[7] = {&a.f1, &a.f2, &a.f3, &a.ksq, &a.u, &a.cp, &a.dp};
+  for (int k = 0; k < 7; ++k) *cols[k] = base + (own ? o_dcol[k] : o_col[k]);
+  a.tll = tll_d;
+  a.cpl = cpl_d;
+  a.team_rays = nullptr;
+  a.team_nrays = 0;
+  const int block = 64;
+  const int grid = (nrays + block - 1) / block;
+  // self-starter in FP64 (cpr_selfs), the march (cpr_march) and the TL
+  // post-pass (cpr_read_outputs), with the same kernels as a plan's
+  hipLaunchKernelGGL(k_selfs, dim3(grid), dim3(block), 0, st, a);
+  HIPCHK(hipGetLastError());
+  const cx<double> *ud = static_cast<const cx<double> *>(a.u);
+  for (int k = 0; k < 7; ++k) *cols[k] = base + o_col[k];
+  if (own) {
+    hipLaunchKernelGGL(k_field_from_starter<float>, dim3(grid), dim3(block), 0, st, a, ud);
+    HIPCHK(hipGetLastError());
+  }
+  if (max_steps > 0) {
+    if (precision == CPR_C64)
+      hipLaunchKernelGGL(k_march_thread<float>, dim3(grid), dim3(block), 0, st, a, 0, max_steps);
+    else
+      hipLaunchKernelGGL(k_march_thread<double>, dim3(grid), dim3(block), 0, st, a, 0, max_steps);
+    HIPCHK(hipGetLastError());
+  }
+  if (nvr_max > 0) {
+    hipLaunchKernelGGL(k_tl_post, dim3(grid, nvr_max), dim3(block), 0, st, a);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}

@@ -138,6 +138,29 @@ int cpr_team_shape(int precision, int nz, int *rows, int *lanes);
 /* Kernel variant and team shape chosen for ray b (for tests/autotuning). */
 int cpr_plan_describe(cpr_plan *p, int b, int *variant, int *team_rows, int *team_lanes);
 
+/* Bytes of the device array behind cpr_march_noalloc (one per GPU). */
+#define CPR_NOALLOC_SCRATCH_BYTES (64LL << 20)
+/* Self-starter, the whole march and the TL line of every ray in one call
+ * that allocates no device memory.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It runs the kernels of a
+ * thread-variant plan (cpr_plan_create with CPR_VARIANT_THREAD, cpr_selfs,
+ * cpr_march over every step, the TL of cpr_read_outputs) and gives the same
+ * results; the ray tables, the march state and the column workspace live in
+ * one library-owned device array of CPR_NOALLOC_SCRATCH_BYTES per GPU, held
+ * from the library's load to the end of the process.  Inputs are host arrays
+ * laid out as for cpr_plan_create (copied to the device on `stream`);
+ * outputs are the caller's device arrays, range-major: tll_d
+ * [nvr_max][nrays] f64 and cpl_d [nvr_max][nrays] complex128 (interleaved
+ * re, im); entries past a ray's nvr are set to zero.  Asynchronous on
+ * `stream` (a hipStream_t; NULL = the default stream).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other host threads) race.  A batch that needs more than the array holds
+ * (about 132 B per row and ray in complex64, 88 B in complex128, plus the
+ * tables) is rejected with an error that states its need. */
+int cpr_march_noalloc(int precision, int nrays, const cpr_ray *rays, const cpr_env *envs, const double *env_data,
+                      int64_t env_len, const double *pade, int nvr_max, double *tll_d, double *cpl_d, void *stream);
+
 /* ------------------------------------------------------------------------
  * Learned TL surrogate (a 1-D Fourier-operator network over depth).  These
  * have no counterpart in the reference: they are the training and inference
