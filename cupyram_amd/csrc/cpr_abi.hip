@@ -21,6 +21,8 @@ __global__ void k_field_from_starter(MarchArgs a, const cx<double> *ud);
 __global__ void k_selfs(MarchArgs a);
 __global__ void k_tl_post(MarchArgs a);
 int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, int s1, hipStream_t st);
+template <class R>
+__global__ void k_team_terms(MarchArgs a, R *qtab);
 bool team_shape_supported(int precision, int rows, int lanes);
 void team_auto_shape(int precision, int nz, int *rows, int *lanes);
 int64_t team_dbuf_bytes(int precision, int rows, int lanes, int nrays);
@@ -85,6 +87,7 @@ struct cpr_plan {
   int32_t *team_rays_d = nullptr;
   char *dbuf_d = nullptr;     // team coefficients (all team shapes)
   double *mbuf_d = nullptr;   // team media cache
+  char *qtab_d = nullptr;     // team per-term constants
   double *stage_d = nullptr;  // staging buffer of the ray-major output reads
   int64_t stage_n = 0;        // its size in doubles
   int variant = CPR_VARIANT_THREAD;
@@ -130,6 +133,7 @@ struct cpr_plan {
     a.team_nrays = nrays;
     a.dbuf = dbuf_d;
     a.mbuf = mbuf_d;
+    a.qtab = qtab_d;
     a.team_rows = variant == CPR_VARIANT_TEAM ? team_rows : 0;
     a.team_lanes = variant == CPR_VARIANT_TEAM ? team_lanes : 0;
     return a;
@@ -306,7 +310,7 @@ int cpr_plan_destroy(cpr_plan *p) {
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
                   static_cast<void *>(p->cpl_d), static_cast<void *>(p->tlg_d), static_cast<void *>(p->cpg_d),
                   static_cast<void *>(p->team_rays_d), static_cast<void *>(p->dbuf_d),
-                  static_cast<void *>(p->mbuf_d), static_cast<void *>(p->stage_d),
+                  static_cast<void *>(p->mbuf_d), static_cast<void *>(p->qtab_d), static_cast<void *>(p->stage_d),
                   static_cast<void *>(p->cells_d),
                   static_cast<void *>(p->tlcell_d)})
     if (q) (void)hipFree(q);
@@ -408,9 +412,23 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
   if (hipMemcpyAsync(p->rays_d, rays, sizeof(cpr_ray) * cfg->nrays, hipMemcpyHostToDevice, p->stream) ||
       hipMemcpyAsync(p->envs_d, envs, sizeof(cpr_env) * cfg->nrays, hipMemcpyHostToDevice, p->stream) ||
       hipMemcpyAsync(p->env_d, env_data, sizeof(double) * env_len, hipMemcpyHostToDevice, p->stream) ||
-      hipMemcpyAsync(p->pade_d, pade, sizeof(double) * npade, hipMemcpyHostToDevice, p->stream) ||
-      hipStreamSynchronize(p->stream))
+      hipMemcpyAsync(p->pade_d, pade, sizeof(double) * npade, hipMemcpyHostToDevice, p->stream))
     return bail(fail("upload failed"));
+  if (p->variant == CPR_VARIANT_TEAM) {
+    // per-term constants of the team march, formed once on the device
+    const int64_t nq = static_cast<int64_t>(cfg->nrays) * 3 * np_ * 6;
+    if (dalloc(p, &p->qtab_d, static_cast<size_t>(nq) * p->real_size())) return bail(-1);
+    const MarchArgs m = p->args();
+    const int block = 64, grid = (cfg->nrays + block - 1) / block;
+    if (cfg->precision == CPR_C64)
+      hipLaunchKernelGGL(k_team_terms<float>, dim3(grid), dim3(block), 0, p->stream, m,
+                         reinterpret_cast<float *>(p->qtab_d));
+    else
+      hipLaunchKernelGGL(k_team_terms<double>, dim3(grid), dim3(block), 0, p->stream, m,
+                         reinterpret_cast<double *>(p->qtab_d));
+    if (hipGetLastError() != hipSuccess) return bail(fail("k_team_terms launch failed"));
+  }
+  if (hipStreamSynchronize(p->stream) != hipSuccess) return bail(fail("upload failed"));
   *out = p;
   return 0;
 }

@@ -60,6 +60,7 @@ struct MarchArgs {
   void *dbuf;                // Galerkin coefficients of every team row (streamed
                              // by deep teams; saved between launches by the others)
   void *mbuf;                // FP64 water and bottom media of every team row
+  const void *qtab;          // team per-term constants [nrays][3 sets][np][q, 4q, rho] in R
   // layout of the field u: team_rows = 0: depth-major [row][ld]; > 0: the
   // team layout [ray][k][lane] of team_rows rows x team_lanes lanes
   int team_rows, team_lanes;

@@ -91,71 +91,77 @@ __device__ inline R shfl_down_r(R v, int s, int w) {
 // the team.  TW > 0: wave shuffles of width TW.  TW == 0: LDS with one
 // barrier per exchange.  LDS layout: two buffers, used alternately, so a
 // buffer is written again only after every lane has passed the barrier that
-// follows its last reads of it; each buffer holds three arrays of
-// kPad + T + kPad elements whose kPad-element margins are zeroed once
-// (team_lds_init) and never written, so reads at distance s <= kPad need no
-// bounds test.  Wider distances (PCR levels s >= 64) are wave-uniform tests.
+// follows its last reads of it.  A buffer is an array of per-lane slots of
+// three complex values (one exchange writes one slot per lane, so reading a
+// neighbour's values takes one address), with zeroed margins of kPad slots on
+// either side: reads at distance s <= kPad need no bounds test, and at the
+// wider PCR distances (s = 64, 128, .. with T a multiple of 64) whether lane
+// t +- s lies in the team is the same for every lane of a wavefront, a
+// scalar branch.  The margins are zeroed once per launch (init) and never
+// written.
 constexpr int kPad = 32;
 
 template <class R>
-__device__ inline cx<R> lds_or_zero(const cx<R> *B, int i, int t, bool ok) {
-  const cx<R> v = B[ok ? i : t];
-  return ok ? v : cxm<R>(0, 0);
-}
+struct Slot {
+  cx<R> a, c, f;
+};
 
 // LDS bytes of an LDS-mode team of T lanes
 template <class R>
 constexpr size_t team_lds_bytes(int T) {
-  return 2 * 3 * static_cast<size_t>(T + 2 * kPad) * sizeof(cx<R>);
+  return 2 * static_cast<size_t>(T + 2 * kPad) * sizeof(Slot<R>);
 }
 
 template <class R, int TW>
 struct Team {
   int t, T;
-  cx<R> *lds;  // LDS mode: 2 buffers x 3 arrays x (kPad + T + kPad)
+  Slot<R> *lds;  // LDS mode: 2 buffers of (kPad + T + kPad) slots
   int ph;
 
   // zero both buffers (margins included); one barrier
   __device__ void init() {
     if constexpr (TW == 0) {
-      const int n = 2 * 3 * (T + 2 * kPad);
-      for (int i = t; i < n; i += T) lds[i] = cxm<R>(0, 0);
+      const int n = 2 * (T + 2 * kPad);
+      for (int i = t; i < n; i += T) lds[i] = Slot<R>{cxm<R>(0, 0), cxm<R>(0, 0), cxm<R>(0, 0)};
       __syncthreads();
     }
   }
-  // data start of array 0 of the next buffer; array j starts at + j * stride()
-  __device__ cx<R> *next_buf() {
-    cx<R> *b = lds + ph * 3 * (T + 2 * kPad) + kPad;
+  // lane 0's slot of the next buffer
+  __device__ Slot<R> *next_buf() {
+    Slot<R> *b = lds + ph * (T + 2 * kPad) + kPad;
     ph ^= 1;
     return b;
   }
-  __device__ int stride() const { return T + 2 * kPad; }
 
-  __device__ cx<R> up1(cx<R> v, int s) {
+  // y_{t-1} and y_{t+1} of a value y (zero beyond the team's ends)
+  __device__ void neighbours(cx<R> v, cx<R> &from_prev, cx<R> &from_next) {
     if constexpr (TW > 0) {
-      cx<R> r = cxm<R>(shfl_up_r(v.re, s, TW), shfl_up_r(v.im, s, TW));
-      return t >= s ? r : cxm<R>(0, 0);
+      const cx<R> r = cxm<R>(shfl_up_r(v.re, 1, TW), shfl_up_r(v.im, 1, TW));
+      const cx<R> d = cxm<R>(shfl_down_r(v.re, 1, TW), shfl_down_r(v.im, 1, TW));
+      from_prev = t >= 1 ? r : cxm<R>(0, 0);
+      from_next = t + 1 < T ? d : cxm<R>(0, 0);
     } else {
-      cx<R> *B = next_buf();
-      B[t] = v;
+      Slot<R> *B = next_buf();
+      B[t].a = v;
       __syncthreads();
-      return B[t - s];  // s = 1 here: the margin supplies the zero
+      from_prev = B[t - 1].a;
+      from_next = B[t + 1].a;
     }
   }
   // halo: previous lane's `a` and next lane's `b`
   __device__ void halo(cx<R> a, cx<R> b, cx<R> &from_prev, cx<R> &from_next) {
     if constexpr (TW > 0) {
-      from_prev = up1(a, 1);
-      cx<R> r = cxm<R>(shfl_down_r(b.re, 1, TW), shfl_down_r(b.im, 1, TW));
-      from_next = t + 1 < T ? r : cxm<R>(0, 0);
+      const cx<R> r = cxm<R>(shfl_up_r(a.re, 1, TW), shfl_up_r(a.im, 1, TW));
+      const cx<R> d = cxm<R>(shfl_down_r(b.re, 1, TW), shfl_down_r(b.im, 1, TW));
+      from_prev = t >= 1 ? r : cxm<R>(0, 0);
+      from_next = t + 1 < T ? d : cxm<R>(0, 0);
     } else {
-      cx<R> *B = next_buf();
-      const int S = stride();
-      B[t] = a;
-      B[S + t] = b;
+      Slot<R> *B = next_buf();
+      B[t].a = a;
+      B[t].c = b;
       __syncthreads();
-      from_prev = B[t - 1];
-      from_next = B[S + t + 1];
+      from_prev = B[t - 1].a;
+      from_next = B[t + 1].c;
     }
   }
   __device__ void next3(cx<R> a, cx<R> b, cx<R> c, cx<R> &na, cx<R> &nb, cx<R> &nc) {
@@ -169,23 +175,21 @@ struct Team {
       nb = ok ? rb : z;
       nc = ok ? rc : z;
     } else {
-      cx<R> *B = next_buf();
-      const int S = stride();
-      B[t] = a;
-      B[S + t] = b;
-      B[2 * S + t] = c;
+      Slot<R> *B = next_buf();
+      B[t] = Slot<R>{a, b, c};
       __syncthreads();
-      na = B[t + 1];
-      nb = B[S + t + 1];
-      nc = B[2 * S + t + 1];
+      const Slot<R> n = B[t + 1];
+      na = n.a;
+      nb = n.c;
+      nc = n.f;
     }
   }
   // one PCR exchange: (a, c, f) of lanes t-s and t+s
   __device__ void pcr(int s, cx<R> a, cx<R> c, cx<R> f, cx<R> &am, cx<R> &cm, cx<R> &fm, cx<R> &ap, cx<R> &cp,
                       cx<R> &fp) {
-    const bool lo = t >= s, hi = t + s < T;
-    const cx<R> z = cxm<R>(0, 0);
     if constexpr (TW > 0) {
+      const bool lo = t >= s, hi = t + s < T;
+      const cx<R> z = cxm<R>(0, 0);
       const cx<R> a1 = cxm<R>(shfl_up_r(a.re, s, TW), shfl_up_r(a.im, s, TW));
       const cx<R> c1 = cxm<R>(shfl_up_r(c.re, s, TW), shfl_up_r(c.im, s, TW));
       const cx<R> f1 = cxm<R>(shfl_up_r(f.re, s, TW), shfl_up_r(f.im, s, TW));
@@ -199,27 +203,30 @@ struct Team {
       cp = hi ? c2 : z;
       fp = hi ? f2 : z;
     } else {
-      cx<R> *B = next_buf();
-      const int S = stride();
-      B[t] = a;
-      B[S + t] = c;
-      B[2 * S + t] = f;
+      Slot<R> *B = next_buf();
+      B[t] = Slot<R>{a, c, f};
       __syncthreads();
+      Slot<R> m, q;
       if (s <= kPad) {
-        am = B[t - s];
-        cm = B[S + t - s];
-        fm = B[2 * S + t - s];
-        ap = B[t + s];
-        cp = B[S + t + s];
-        fp = B[2 * S + t + s];
+        m = B[t - s];  // the margins hold zeros past the ends
+        q = B[t + s];
       } else {
-        am = lds_or_zero(B, t - s, t, lo);
-        cm = lds_or_zero(B + S, t - s, t, lo);
-        fm = lds_or_zero(B + 2 * S, t - s, t, lo);
-        ap = lds_or_zero(B, t + s, t, hi);
-        cp = lds_or_zero(B + S, t + s, t, hi);
-        fp = lds_or_zero(B + 2 * S, t + s, t, hi);
+        const Slot<R> z{cxm<R>(0, 0), cxm<R>(0, 0), cxm<R>(0, 0)};
+        if (__builtin_amdgcn_readfirstlane(t >= s))
+          m = B[t - s];
+        else
+          m = z;
+        if (__builtin_amdgcn_readfirstlane(t + s < T))
+          q = B[t + s];
+        else
+          q = z;
       }
+      am = m.a;
+      cm = m.c;
+      fm = m.f;
+      ap = q.a;
+      cp = q.c;
+      fp = q.f;
     }
   }
 };
@@ -536,7 +543,7 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coe
 }
 
 template <class R, int M, int TW, bool DS>
-__device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *lds, int s0, int s1) {
+__device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R> *lds, int s0, int s1) {
   const cpr_ray P = make_uniform(a.rays[b]);
   const cpr_env E = make_uniform(a.envs[b]);
   RayState st = make_uniform(a.state[b]);
@@ -569,6 +576,11 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
   const int ir = static_cast<int>(floor(ri));
   const double dir = ri - ir;
   const int end = s1 < P.nsteps ? s1 : P.nsteps;
+  // halo rows: the previous lane's last row and the next lane's first,
+  // exchanged once per launch; after each factor they come from its back
+  // substitution (the field does not change between steps otherwise)
+  cx<R> um, u_next;
+  team.halo(L.u[M - 1], L.u[0], um, u_next);
   // The media and the coefficients persist between launches: they are
   // rebuilt in full only after the self-starter (dirty) and at profile
   // switches; VGPR-resident coefficients come back from the team buffer.
@@ -593,25 +605,16 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
     }
     // every row stencil of this wavefront above the seabed row: real coefficients
     const bool wreal = __builtin_amdgcn_ballot_w64(g0 + M > st.iz) == 0;
-    const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + st.pset) * 2 * P.np * 2;
+    // Pade factor (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1],
+    // divided by p2: (q [1 4 1] + D) x = (q [1 4 1] + rho D) u with
+    // q = k0^2 / (6 p2), rho = p1 / p2, formed once per plan in FP64
+    // (k_team_terms) and read here through scalar loads.
+    const R *qt = static_cast<const R *>(a.qtab) + (static_cast<int64_t>(b) * 3 + st.pset) * P.np * 6;
     for (int j = 0; j < P.np; ++j) {
-      // Pade factor (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1],
-      // divided by p2: (q [1 4 1] + D) x = (q [1 4 1] + rho D) u with
-      // q = k0^2 / (6 p2), rho = p1 / p2 (formed in FP64).
-      cx<R> q, q4, rho;
-      {
-        const double p1r = pd[2 * j], p1i = pd[2 * j + 1];
-        const double p2r = pd[2 * (P.np + j)], p2i = pd[2 * (P.np + j) + 1];
-        const double n2 = 1.0 / (p2r * p2r + p2i * p2i);
-        const double ir = p2r * n2, ii = -p2i * n2;
-        const double a1d = P.k0 * P.k0 / 6.0;
-        q = cxm<R>(static_cast<R>(a1d * ir), static_cast<R>(a1d * ii));
-        q4 = cxm<R>(static_cast<R>(4.0 * a1d * ir), static_cast<R>(4.0 * a1d * ii));
-        rho = cxm<R>(static_cast<R>(p1r * ir - p1i * ii), static_cast<R>(p1r * ii + p1i * ir));
-      }
-      // 1. halo rows and downward elimination with the spike
-      cx<R> um, u_next;
-      team.halo(L.u[M - 1], L.u[0], um, u_next);
+      const cx<R> q = cxm<R>(qt[6 * j], qt[6 * j + 1]);
+      const cx<R> q4 = cxm<R>(qt[6 * j + 2], qt[6 * j + 3]);
+      const cx<R> rho = cxm<R>(qt[6 * j + 4], qt[6 * j + 5]);
+      // 1. downward elimination with the spike
       if (wreal)
         eliminate_down<true>(L, X, C, q, q4, rho, um, u_next, mv);
       else
@@ -649,12 +652,17 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
         ec = cn * ib;
         ef = fn * ib;
       }
-      // 4. back substitution (rows past nz come out zero)
+      // 4. back substitution (rows past nz come out zero).  The next lane's
+      // new first row is formed here from its (D, W, C) of step 3 exactly as
+      // that lane forms it, so the next factor needs no halo exchange.
       const cx<R> yc = ef;
-      const cx<R> yl = team.up1(yc, 1);
+      cx<R> yl, yr;
+      team.neighbours(yc, yl, yr);
 #pragma unroll
       for (int k = 0; k < M - 1; ++k) L.u[k] = cfms(cfms(L.u[k], X.w(k), yl), X.c(k), yc);
       L.u[M - 1] = yc;
+      um = yl;
+      u_next = cfms(cfms(nD, nW, yc), nC, yr);
     }
     if (a.cells) {
       const int bin = st.tlc + 1;
@@ -669,9 +677,17 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
       if (st.tlc < P.nvr) {
         const int ta = ir >= 1 ? (ir - 1) / M : 0, ka = ir >= 1 ? (ir - 1) % M : 0;
         const int tb = ir / M, kb = ir % M;
-        cx<R> va = pick<R, M>(L.u, ka), vb = pick<R, M>(L.u, kb);
-        const R fa = static_cast<R>(f3_from_cache<M>(mc, ka, g0, st.iz));
-        const R fb = static_cast<R>(f3_from_cache<M>(mc, kb, g0, st.iz));
+        // only the lanes that own rows ir and ir+1 pick them out
+        cx<R> va = cxm<R>(0, 0), vb = cxm<R>(0, 0);
+        R fa = R(0), fb = R(0);
+        if (t == ta) {
+          va = pick<R, M>(L.u, ka);
+          fa = static_cast<R>(f3_from_cache<M>(mc, ka, g0, st.iz));
+        }
+        if (t == tb) {
+          vb = pick<R, M>(L.u, kb);
+          fb = static_cast<R>(f3_from_cache<M>(mc, kb, g0, st.iz));
+        }
         cx<R> ua, ub;
         R f3a, f3b;
         if constexpr (TW > 0) {
@@ -681,20 +697,20 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
           ub = cxm<R>(__shfl(vb.re, tbs, TW), __shfl(vb.im, tbs, TW));
           f3b = __shfl(fb, tbs, TW);
         } else {
-          cx<R> *B = team.next_buf();
+          Slot<R> *B = team.next_buf();
           if (t == ta) {
-            B[0] = va;
-            B[2] = cxm<R>(fa, 0);
+            B[0].a = va;
+            B[0].f = cxm<R>(fa, 0);
           }
           if (t == tb) {
-            B[1] = vb;
-            B[3] = cxm<R>(fb, 0);
+            B[0].c = vb;
+            B[1].f = cxm<R>(fb, 0);
           }
           __syncthreads();
-          ua = B[0];
-          ub = B[1];
-          f3a = B[2].re;
-          f3b = B[3].re;
+          ua = B[0].a;
+          ub = B[0].c;
+          f3a = B[0].f.re;
+          f3b = B[1].f.re;
         }
         if (ir < 1) {
           ua = cxm<R>(0, 0);
@@ -739,6 +755,37 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, cx<R> *
 
 }  // namespace
 
+// Per-term constants of the team march for every ray, Pade set and term,
+// formed in FP64 and rounded to the march precision: the factor
+// (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1] divided by p2 is
+// (q [1 4 1] + D) x = (q [1 4 1] + rho D) u, q = k0^2 / (6 p2), rho = p1 / p2;
+// stored as q, 4 q, rho (re, im each).
+template <class R>
+__global__ void k_team_terms(MarchArgs a, R *qtab) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.nrays) return;
+  const cpr_ray P = a.rays[b];
+  for (int set = 0; set < 3; ++set) {
+    const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + set) * 2 * P.np * 2;
+    R *o = qtab + (static_cast<int64_t>(b) * 3 + set) * P.np * 6;
+    for (int j = 0; j < P.np; ++j) {
+      const double p1r = pd[2 * j], p1i = pd[2 * j + 1];
+      const double p2r = pd[2 * (P.np + j)], p2i = pd[2 * (P.np + j) + 1];
+      const double n2 = 1.0 / (p2r * p2r + p2i * p2i);
+      const double ir = p2r * n2, ii = -p2i * n2;
+      const double a1d = P.k0 * P.k0 / 6.0;
+      o[6 * j + 0] = static_cast<R>(a1d * ir);
+      o[6 * j + 1] = static_cast<R>(a1d * ii);
+      o[6 * j + 2] = static_cast<R>(4.0 * a1d * ir);
+      o[6 * j + 3] = static_cast<R>(4.0 * a1d * ii);
+      o[6 * j + 4] = static_cast<R>(p1r * ir - p1i * ii);
+      o[6 * j + 5] = static_cast<R>(p1r * ii + p1i * ir);
+    }
+  }
+}
+template __global__ void k_team_terms<float>(MarchArgs, float *);
+template __global__ void k_team_terms<double>(MarchArgs, double *);
+
 // Shuffle mode: one 64-lane team per wavefront, four per 256-thread block.
 // At least two waves per SIMD (256 VGPRs): a lone wave cannot hide the
 // latency of its shuffles and reciprocals (measured 1.6x on 8- and 12-row
@@ -758,7 +805,7 @@ __global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s
   const int team = blockIdx.x;
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 0, DS>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<cx<R> *>(smem), s0, s1);
+  march_team_body<R, M, 0, DS>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<Slot<R> *>(smem), s0, s1);
 }
 
 namespace {

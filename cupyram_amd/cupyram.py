@@ -17,7 +17,10 @@ Deliberate differences from the reference, each a fix of a reference defect:
   * 'Proc Time' is wall-clock time of run() (the reference reports CPU
     process time, cupyram.py:288,320);
   * the self-starter's (1-X)^-1 smoothing applies one factor as PyRAM does
-    (the reference's fused kernel also sweeps np-1 identity factors).
+    (the reference's fused kernel also sweeps np-1 identity factors);
+  * the CP Grid is reshaped and squeezed exactly like the TL Grid (the
+    reference leaves it [N_env, 1, nvz, nvr] for one frequency and squeezes
+    the wrong axis for one environment, cupyram.py:356-368).
 
 Extensions (keyword arguments the reference does not have):
   precision='c128'|'c64'   arithmetic of the march (FP64 Pade + starter in both)
