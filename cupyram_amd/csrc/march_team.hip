@@ -640,7 +640,11 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
         ec = ec * ib;
         ef = ef * ib;
       }
-      for (int s = 1; s < T; s <<= 1) {
+      // unrolled so that each level's distance s is a constant (T <= 512)
+#pragma unroll
+      for (int l = 0; l < 9; ++l) {
+        const int s = 1 << l;
+        if (s >= T) break;
         cx<R> am, cm, fm, ap, cp, fp;
         team.pcr(s, ea, ec, ef, am, cm, fm, ap, cp, fp);
         const cx<R> bn = cfms(cfms(cxm<R>(1, 0), ea, cm), ec, ap);
