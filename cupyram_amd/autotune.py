@@ -11,7 +11,7 @@ the library's static cost model (``cpr_team_shape``) when no table exists.
 Every candidate marches the same rays from the same start field; results are
 not compared here (the GPU tests cover every shape against the oracle).
 
-Usage: ``python -m cupyram_amd.autotune [--steps 20] [--out path]``.
+Usage: ``python -m cupyram_amd.autotune [--steps 40] [--out path]``.
 """
 from __future__ import annotations
 
@@ -85,7 +85,7 @@ def _synthetic(nz: int, nrays: int, rmax: float = 600.0):
                 compute_grids=False)
 
 
-def time_shape(model, shape, steps, warmup=3):
+def time_shape(model, shape, steps, warmup=10):
     plan = model._make_plan(np.arange(model._total_batch), shape=shape, variant="team")
     try:
         plan.selfs()
@@ -101,7 +101,7 @@ def time_shape(model, shape, steps, warmup=3):
     return cells / dt / 1e6
 
 
-def sweep(precisions=(64, 128), nz_list=NZ_SWEEP, steps=20, lanes_target=4 * 131072, log=print):
+def sweep(precisions=(64, 128), nz_list=NZ_SWEEP, steps=40, lanes_target=4 * 131072, log=print):
     from .cupyram import CuPyRAM
     table = {}
     for prec in precisions:
@@ -133,7 +133,7 @@ def sweep(precisions=(64, 128), nz_list=NZ_SWEEP, steps=20, lanes_target=4 * 131
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--out", default=TABLE)
     ap.add_argument("--precision", type=int, nargs="*", default=[64, 128])
     args = ap.parse_args(argv)
