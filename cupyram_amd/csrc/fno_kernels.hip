@@ -388,7 +388,8 @@ __global__ void k_cast_bf16_t(const float *x, bf16 *y, int R, int Cc) {
 //            M the mixed modes [K][S][2C] with k' < K the real and k' >= K
 //            the imaginary parts; Bsyn = hi + lo bf16 split of the FP32
 //            synthesis matrix, so the product keeps ~16 mantissa bits)
-//     hout = ACT ? GELU(acc + bias) : acc + bias   (bf16)
+//     hout = ACT ? GELU(acc + bias) : acc + bias   (bf16; acc + bias also to
+//            pre in FP32 when given, for the backward pass of training)
 //     xh[k'][s][c] = sum_z Ban[k'][z] hout[s][z][c] (ANA: the next layer's
 //            truncated DFT, Ban = hi + lo split of the analysis matrix)
 //   k_fno_project: y[row] = GELU(h . P1 + c1) . P2 + c2 (f32)
@@ -413,6 +414,7 @@ struct LayerArgs {
   const bf16 *ban;   // [2][2K][Z] hi, lo
   bf16 *hout;        // [S][Z][C]
   bf16 *xh;          // [K][S][2C] (ANA)
+  float *pre;        // [S][Z][C] FP32 pre-activation acc + bias, or null (training)
   int S;
 };
 
@@ -501,6 +503,7 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
       for (int r = 0; r < 16; ++r) {
         const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         float v = acc[i][j][r] + bn;
+        if (g.pre && live) g.pre[(static_cast<int64_t>(s) * FZ + z) * FC + n] = v;
         if (ACT) v = gelu_f(v);
         const bf16 hv = static_cast<bf16>(v);
         if (live) g.hout[(static_cast<int64_t>(s) * FZ + z) * FC + n] = hv;
@@ -765,8 +768,8 @@ extern "C" int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *strea
 }
 
 extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const void *wt, const float *bias,
-                             const void *mh, const void *bsyn, const void *ban, void *hout, void *xh, int S,
-                             void *stream) {
+                             const void *mh, const void *bsyn, const void *ban, void *hout, void *xh, float *pre,
+                             int S, void *stream) {
   if (S <= 0 || !hin || !wt || !hout || (syn && (!mh || !bsyn)) || (ana && (!ban || !xh)))
     return cpr_fail("cpr_fno_layer: invalid arguments");
   LayerArgs g;
@@ -778,6 +781,7 @@ extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const v
   g.ban = static_cast<const bf16 *>(ban);
   g.hout = static_cast<bf16 *>(hout);
   g.xh = static_cast<bf16 *>(xh);
+  g.pre = pre;
   g.S = S;
   const dim3 grid((S + 1) / 2);
   hipStream_t st = static_cast<hipStream_t>(stream);

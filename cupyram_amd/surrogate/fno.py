@@ -124,7 +124,8 @@ class FNO:
         """Refresh the BF16 copies from the FP32 master weights."""
         for n, t in self.bf.items():
             k.cast_bf16(self.params[n], t)
-        self._fused = None  # transposed weights of the fused kernels, rebuilt on next use
+        self._fused = None  # output bias of the fused projection, read again on next use
+        self._wt = None     # transposed weights of the fused kernels, rebuilt on next use
         for l in range(1, self.cfg.layers + 1):
             k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes, self.cfg.width)
 
@@ -149,8 +150,25 @@ class FNO:
         self._bufs.clear()
 
     # -- forward ----------------------------------------------------------
+    def _transposed(self):
+        """bf16 [out][in] copies of the lift, pointwise and first projection
+        weights (the fused kernels' operand layout), rebuilt after a step."""
+        if self._wt is None or self._wt[0] != self.step_count:
+            wt = {}
+            for n in self.bf:
+                if n.startswith(("W0", "Wp", "P1")):
+                    w = self.params[n]
+                    wt[n] = torch.empty(w.shape[1], w.shape[0], dtype=torch.bfloat16, device=self.device)
+                    k.cast_bf16_t(w, wt[n])
+            self._wt = (self.step_count, wt)
+        return self._wt[1]
+
     def forward(self, x, train: bool = False):
-        """x: bf16 [S, Z, F] -> prediction f32 [S*Z] (normalised TL)."""
+        """x: bf16 [S, Z, F] -> prediction f32 [S*Z] (normalised TL).  With
+        train=True the values the backward pass needs are kept; networks of
+        the fused shape then run their lift and layers through the fused
+        kernels (which also store the pre-activations), the others and
+        inference through the layer-by-layer kernels."""
         cfg = self.cfg
         S, Z, F = x.shape
         if Z != cfg.depth_points or F != cfg.features or x.dtype != torch.bfloat16:
@@ -161,6 +179,8 @@ class FNO:
         bf, f32 = torch.bfloat16, torch.float32
         self._S = S
         self._x = x
+        if train and self.fusable:
+            return self._forward_train_fused(x, S)
         h = [self._buf("h0", (R, Cw), bf)]
         k.gemm(x, self.bf["W0"], h[0], R, Cw, F, bias=self.params["b0"], epi=k.EPI_BF16)
         y = self._buf("y", (R, Cw), f32)
@@ -184,6 +204,38 @@ class FNO:
         self._h = h
         return pred
 
+    def _forward_train_fused(self, x, S):
+        """The training forward pass on the fused kernels: per layer one
+        mode-mixing product and one fused layer (pointwise product, spectral
+        synthesis, bias, GELU with its FP32 pre-activation stored, the next
+        layer's analysis); the projection layer by layer (its GELU
+        pre-activation and output are kept too)."""
+        cfg = self.cfg
+        Z, Cw, Km, L = cfg.depth_points, cfg.width, cfg.modes, cfg.layers
+        R = S * Z
+        bf, f32 = torch.bfloat16, torch.float32
+        wt = self._transposed()
+        h = [self._buf("h0", (R, Cw), bf)]
+        xh = self._buf("xh1", (Km, S, 2 * Cw), bf)
+        k.fno_layer(x, wt["W0"], self.params["b0"], h[0], S, ban=self.ban, xh=xh, act=False)
+        mh = self._buf("mh", (Km, S, 2 * Cw), bf)
+        for l in range(1, L + 1):
+            k.gemm(xh, self.blk[l], mh, S, 2 * Cw, 2 * Cw, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+                   sc=S * 2 * Cw, epi=k.EPI_BF16)
+            last = l == L
+            xh = None if last else self._buf(f"xh{l + 1}", (Km, S, 2 * Cw), bf)
+            hn = self._buf(f"h{l}", (R, Cw), bf)
+            k.fno_layer(h[-1], wt[f"Wp{l}"], self.params[f"bp{l}"], hn, S, mh=mh, bsyn=self.bsyn,
+                        ban=None if last else self.ban, xh=xh, act=True, pre=self._buf(f"pre{l}", (R, Cw), f32))
+            h.append(hn)
+        q = self._buf("q", (R, Cw), bf)
+        preq = self._buf("preq", (R, Cw), f32)
+        k.gemm(h[-1], self.bf["P1"], q, R, Cw, Cw, bias=self.params["c1"], pre=preq, epi=k.EPI_GELU_BF16)
+        pred = self._buf("pred", (R,), f32)
+        k.gemm(q, self.bf["P2"], pred, R, 1, Cw, bias=self.params["c2"], epi=k.EPI_F32)
+        self._h = h
+        return pred
+
     def forward_fused(self, x):
         """Inference with the fused kernels: per layer one mode-mixing product
         and one fused layer kernel (pointwise product, spectral synthesis,
@@ -198,15 +250,9 @@ class FNO:
         x = x.contiguous()
         bf = torch.bfloat16
         if self._fused is None or self._fused[0] != self.step_count:
-            # transposed ([out][in]) bf16 products and the output bias as a host float
-            wt = {}
-            for n in self.bf:
-                if n.startswith(("W0", "Wp", "P1")):
-                    w = self.params[n]
-                    wt[n] = torch.empty(w.shape[1], w.shape[0], dtype=bf, device=self.device)
-                    k.cast_bf16_t(w, wt[n])
-            self._fused = (self.step_count, wt, float(self.params["c2"].item()))
-        wt, c2 = self._fused[1], self._fused[2]
+            # the output bias as a host float
+            self._fused = (self.step_count, float(self.params["c2"].item()))
+        wt, c2 = self._transposed(), self._fused[1]
         Cw, Km, L = cfg.width, cfg.modes, cfg.layers
         R = S * Z
         bf = torch.bfloat16

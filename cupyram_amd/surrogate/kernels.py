@@ -155,9 +155,10 @@ def cast_bf16(x, y):
 FUSED_Z, FUSED_C, FUSED_K = 64, 128, 16
 
 
-def fno_layer(hin, wt, bias, hout, S, *, mh=None, bsyn=None, ban=None, xh=None, act=True):
-    """Fused inference layer (cpr_fno_layer): hout = act(hin W + bias [+ syn(mh)]),
-    and xh = the truncated DFT of hout when ban/xh are given."""
+def fno_layer(hin, wt, bias, hout, S, *, mh=None, bsyn=None, ban=None, xh=None, act=True, pre=None):
+    """Fused layer (cpr_fno_layer): hout = act(hin W + bias [+ syn(mh)]), xh =
+    the truncated DFT of hout when ban/xh are given, and the FP32
+    pre-activation into `pre` when given (training)."""
     Z, Cw, K2 = FUSED_Z, FUSED_C, 2 * FUSED_K
     syn, ana = mh is not None, xh is not None
     _need(hin, torch.bfloat16, S * Z * Cw, "fno_layer hin")
@@ -171,8 +172,10 @@ def fno_layer(hin, wt, bias, hout, S, *, mh=None, bsyn=None, ban=None, xh=None, 
     if ana:
         _need(ban, torch.bfloat16, 2 * K2 * Z, "fno_layer ban")
         _need(xh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer xh")
+    if pre is not None:
+        _need(pre, torch.float32, S * Z * Cw, "fno_layer pre")
     _abi.check(_abi.lib().cpr_fno_layer(int(syn), int(act), int(ana), _p(hin), _p(wt), _p(bias), _p(mh),
-                                        _p(bsyn), _p(ban), _p(hout), _p(xh), int(S), _stream()))
+                                        _p(bsyn), _p(ban), _p(hout), _p(xh), _p(pre), int(S), _stream()))
 
 
 def fno_project(h, p1t, c1, p2, c2, y, S):

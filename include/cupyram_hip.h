@@ -214,10 +214,12 @@ int cpr_fno_fill(float *x, int64_t n, float v, void *stream);
  * imaginary mixed modes mh [16][S][256], bsyn [2][64][32] the hi and lo bf16
  * parts of the synthesis matrix, act = GELU; and (ana) xh [16][S][256] =
  * the truncated DFT of hout with ban [2][32][64] (hi, lo parts of the
- * analysis matrix).  Supported (syn, act, ana): (0,0,1) lift, (1,1,1) inner
- * layer, (1,1,0) last layer. */
+ * analysis matrix).  pre (FP32 [S][64][128], or NULL) receives the
+ * pre-activation hin . W + bias + syn(mh) for the backward pass of training.
+ * Supported (syn, act, ana): (0,0,1) lift, (1,1,1) inner layer, (1,1,0) last
+ * layer. */
 int cpr_fno_layer(int syn, int act, int ana, const void *hin, const void *wt, const float *bias, const void *mh,
-                  const void *bsyn, const void *ban, void *hout, void *xh, int S, void *stream);
+                  const void *bsyn, const void *ban, void *hout, void *xh, float *pre, int S, void *stream);
 /* y [S*64] f32 = GELU(h . P1 + c1) . P2 + c2 with p1t = P1 transposed
  * ([out][in] bf16), p2 [128] f32. */
 int cpr_fno_project(const void *h, const void *p1t, const float *c1, const float *p2, float c2, float *y, int S,
