@@ -100,7 +100,9 @@ _SYMBOLS = {
     "cpr_fno_gelu_bwd": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "cpr_fno_spec_expand": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_spec_fold": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
-    "cpr_fno_colsum": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_colsum": ([C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_outer_gelu_bwd": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_mse": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
                      C.c_void_p, C.c_void_p], C.c_int),
     "cpr_fno_adam": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
@@ -110,6 +112,7 @@ _SYMBOLS = {
     "cpr_fno_cast_bf16_t": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_layer": ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_layer_bwd": ([C.c_void_p] * 9 + [C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_project": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int,
                          C.c_void_p], C.c_int),
 }

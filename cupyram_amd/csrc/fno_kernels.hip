@@ -6,7 +6,7 @@
 //                   four waves of 64x64, v_mfma_f32_32x32x16_bf16
 //   k_spec_analysis modes[k][s][c] = sum_z Bc[k][z] x[s][z][c] (and Bs -> imag)
 //   k_spec_synth    y[s][z][c] (+)= sum_k Bc[k][z] m[k][s][c] + Bs[k][z] m[k][s][C+c]
-//   k_gelu_bwd, k_colsum, k_mse, k_adam, k_fill, k_cast
+//   k_gelu_bwd, k_outer_gelu_bwd, k_colsum, k_mse, k_adam, k_fill, k_cast
 //
 // The surrogate has no counterpart in the reference; it is trained only on TL
 // fields produced by this repository's march.
@@ -286,6 +286,25 @@ __global__ void k_gelu_bwd(const float *dy, const float *pre, bf16 *dx, int64_t 
   dx[i] = static_cast<bf16>(pre ? dy[i] * gelu_grad(pre[i]) : dy[i]);
 }
 
+// dx[s][z][c] = dy[s][z] p[c] GELU'(pre[s][z][c]) (bf16), the gradient of the
+// pre-activation of a GELU layer followed by a 1-column product; part[s][c]
+// = sum_z dx[s][z][c] (f32 bias gradient partials).  One block of C threads
+// per depth column s.
+__global__ void k_outer_gelu_bwd(const bf16 *dy, const float *p, const float *pre, bf16 *dx, float *part, int Z,
+                                 int C) {
+  const int s = blockIdx.x, c = threadIdx.x;
+  if (c >= C) return;
+  const float pc = p[c];
+  float cs = 0.0f;
+  for (int z = 0; z < Z; ++z) {
+    const int64_t row = static_cast<int64_t>(s) * Z + z;
+    const float v = static_cast<float>(dy[row]) * pc * gelu_grad(pre[row * C + c]);
+    dx[row * C + c] = static_cast<bf16>(v);
+    cs += v;
+  }
+  part[static_cast<int64_t>(s) * C + c] = cs;
+}
+
 // Spectral weights: complex W[k] = Wr[k] + i Wi[k] ([K][C][C], row = input
 // channel) acts on the real row vector [Re | Im] as the block matrix
 // [[Wr, Wi], [-Wi, Wr]] ([K][2C][2C]).  expand writes the block (bf16);
@@ -314,8 +333,9 @@ __global__ void k_spec_fold(const float *g, float *gr, float *gi, int K, int C) 
   gi[i] = b[r * L + C + c] - b[(C + r) * L + c];
 }
 
-// out[n] += sum_m x[m][n] (column sums of an M x N bf16 matrix)
-__global__ void k_colsum(const bf16 *x, float *out, int64_t M, int N, int rows_per_block) {
+// out[n] += sum_m x[m][n] (column sums of an M x N bf16 or f32 matrix)
+template <class T>
+__global__ void k_colsum(const T *x, float *out, int64_t M, int N, int rows_per_block) {
   const int n = threadIdx.x + blockIdx.y * blockDim.x;
   if (n >= N) return;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_block;
@@ -392,11 +412,21 @@ __global__ void k_cast_bf16_t(const float *x, bf16 *y, int R, int Cc) {
 //            pre in FP32 when given, for the backward pass of training)
 //     xh[k'][s][c] = sum_z Ban[k'][z] hout[s][z][c] (ANA: the next layer's
 //            truncated DFT, Ban = hi + lo split of the analysis matrix)
+//   k_fno_layer<SYN, false, ANA, true> (BWD), one layer of the backward pass:
+//     acc  = dpre_l[rows] . Wp^T + A^T(dxh)        (W passed as Wp[n][k]; SYN
+//            with Bsyn = the analysis matrix transposed, M = the mixed mode
+//            gradients dxh)
+//     hout = acc * GELU'(pin)                      (the previous layer's
+//            pre-activation gradient, bf16; pin = null: acc, for the lift)
+//     csum[s][c] = sum_z hout[s][z][c]            (f32, bias gradient partials)
+//     xh   = S^T(hout)                             (ANA with Ban = the
+//            synthesis matrix: the previous layer's mode gradients)
 //   k_fno_project: y[row] = GELU(h . P1 + c1) . P2 + c2 (f32)
 //
-// They replace, for inference, the sequence spec_synth -> gemm(GELU, addend)
-// -> spec_analysis of the layer kernels above: the FP32 spectral field y and
-// the separate transforms never reach HBM.
+// They replace the sequence spec_synth -> gemm(GELU, addend) -> spec_analysis
+// of the layer kernels above (and, backward, gemm -> spec_synth -> gelu_bwd
+// -> colsum -> spec_analysis): the FP32 spectral field y and the separate
+// transforms never reach HBM.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -415,6 +445,8 @@ struct LayerArgs {
   bf16 *hout;        // [S][Z][C]
   bf16 *xh;          // [K][S][2C] (ANA)
   float *pre;        // [S][Z][C] FP32 pre-activation acc + bias, or null (training)
+  const float *pin;  // BWD: [S][Z][C] FP32 pre-activation the output gradient passes through, or null
+  float *csum;       // BWD: [S][C] per-column sums of the output (bias gradient partials)
   int S;
 };
 
@@ -437,7 +469,7 @@ __device__ inline void pointwise_tiles(const bf16 *hrows, const bf16 *wt, int wn
 
 }  // namespace
 
-template <bool SYN, bool ACT, bool ANA>
+template <bool SYN, bool ACT, bool ANA, bool BWD = false>
 __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
   __shared__ __attribute__((aligned(16))) bf16 ms[2][FK2][FC];    // mixed modes of the block's two columns
   __shared__ __attribute__((aligned(16))) bf16 ht[2][FC][HT_LD];  // output tile, transposed (channel-major)
@@ -492,23 +524,36 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
       }
     }
   }
-  // epilogue: bias, activation, bf16 output (and its transposed copy for ANA)
+  // epilogue: bias, activation, bf16 output (and its transposed copy for
+  // ANA); BWD: the gradient through GELU' of pin, and the column sums
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int n = wn + j * 32 + (lane & 31);
-    const float bn = g.bias ? g.bias[n] : 0.0f;
+    const float bn = !BWD && g.bias ? g.bias[n] : 0.0f;
+    float cs = 0.0f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t o = (static_cast<int64_t>(s) * FZ + z) * FC + n;
         float v = acc[i][j][r] + bn;
-        if (g.pre && live) g.pre[(static_cast<int64_t>(s) * FZ + z) * FC + n] = v;
-        if (ACT) v = gelu_f(v);
+        if (BWD) {
+          if (g.pin && live) v *= gelu_grad(g.pin[o]);
+          cs += v;
+        } else {
+          if (g.pre && live) g.pre[o] = v;
+          if (ACT) v = gelu_f(v);
+        }
         const bf16 hv = static_cast<bf16>(v);
-        if (live) g.hout[(static_cast<int64_t>(s) * FZ + z) * FC + n] = hv;
+        if (live) g.hout[o] = hv;
         if (ANA) ht[sb][n][z] = live ? hv : static_cast<bf16>(0.0f);
       }
+    if (BWD) {
+      // lanes l and l ^ 32 hold the two z halves of column n
+      cs += __shfl_xor(cs, 32, 64);
+      if (live && lane < 32) g.csum[static_cast<int64_t>(s) * FC + n] = cs;
+    }
   }
   if (ANA) {
     __syncthreads();
@@ -715,13 +760,27 @@ extern "C" int cpr_fno_spec_fold(const float *g, float *gr, float *gi, int K, in
   return launched("cpr_fno_spec_fold");
 }
 
-extern "C" int cpr_fno_colsum(const void *x, float *out, int64_t M, int N, void *stream) {
-  if (M <= 0 || N <= 0) return cpr_fail("cpr_fno_colsum: invalid arguments");
+extern "C" int cpr_fno_colsum(const void *x, int x_is_f32, float *out, int64_t M, int N, void *stream) {
+  if (M <= 0 || N <= 0 || !x || !out) return cpr_fail("cpr_fno_colsum: invalid arguments");
   const int rows = 256;
   const dim3 grid(blocks_for(M, rows), (N + 255) / 256);
-  hipLaunchKernelGGL(k_colsum, grid, dim3(N < 256 ? N : 256), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const bf16 *>(x), out, M, N, rows);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (x_is_f32)
+    hipLaunchKernelGGL(k_colsum<float>, grid, dim3(N < 256 ? N : 256), 0, st, static_cast<const float *>(x), out, M,
+                       N, rows);
+  else
+    hipLaunchKernelGGL(k_colsum<bf16>, grid, dim3(N < 256 ? N : 256), 0, st, static_cast<const bf16 *>(x), out, M,
+                       N, rows);
   return launched("cpr_fno_colsum");
+}
+
+extern "C" int cpr_fno_outer_gelu_bwd(const void *dy, const float *p, const float *pre, void *dx, float *part, int S,
+                                      int Z, int C, void *stream) {
+  if (S <= 0 || Z <= 0 || C <= 0 || C > 1024 || !dy || !p || !pre || !dx || !part)
+    return cpr_fail("cpr_fno_outer_gelu_bwd: invalid arguments");
+  hipLaunchKernelGGL(k_outer_gelu_bwd, dim3(S), dim3(C), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const bf16 *>(dy), p, pre, static_cast<bf16 *>(dx), part, Z, C);
+  return launched("cpr_fno_outer_gelu_bwd");
 }
 
 extern "C" int cpr_fno_mse(const float *pred, int ldp, const float *target, const float *mask, void *dpred,
@@ -782,6 +841,8 @@ extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const v
   g.hout = static_cast<bf16 *>(hout);
   g.xh = static_cast<bf16 *>(xh);
   g.pre = pre;
+  g.pin = nullptr;
+  g.csum = nullptr;
   g.S = S;
   const dim3 grid((S + 1) / 2);
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -794,6 +855,36 @@ extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const v
   else
     return cpr_fail("cpr_fno_layer: unsupported (syn, act, ana) combination");
   return launched("cpr_fno_layer");
+}
+
+extern "C" int cpr_fno_layer_bwd(const void *din, const void *w, const void *dxh, const void *bsyn, const void *ban,
+                                 const float *pin, void *dout, void *dmh, float *csum, int S, void *stream) {
+  const bool syn = dxh != nullptr, ana = dmh != nullptr;
+  if (S <= 0 || !din || !w || !dout || !csum || (syn && !bsyn) || (ana && !ban))
+    return cpr_fail("cpr_fno_layer_bwd: invalid arguments");
+  if (!syn && !ana) return cpr_fail("cpr_fno_layer_bwd: unsupported combination (no dxh and no dmh)");
+  LayerArgs g;
+  g.hin = static_cast<const bf16 *>(din);
+  g.wt = static_cast<const bf16 *>(w);
+  g.bias = nullptr;
+  g.mh = static_cast<const bf16 *>(dxh);
+  g.bsyn = static_cast<const bf16 *>(bsyn);
+  g.ban = static_cast<const bf16 *>(ban);
+  g.hout = static_cast<bf16 *>(dout);
+  g.xh = static_cast<bf16 *>(dmh);
+  g.pre = nullptr;
+  g.pin = pin;
+  g.csum = csum;
+  g.S = S;
+  const dim3 grid((S + 1) / 2);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (syn && ana)
+    hipLaunchKernelGGL((k_fno_layer<true, false, true, true>), grid, dim3(256), 0, st, g);
+  else if (syn)
+    hipLaunchKernelGGL((k_fno_layer<true, false, false, true>), grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((k_fno_layer<false, false, true, true>), grid, dim3(256), 0, st, g);
+  return launched("cpr_fno_layer_bwd");
 }
 
 extern "C" int cpr_fno_project(const void *h, const void *p1t, const float *c1, const float *p2, float c2, float *y,

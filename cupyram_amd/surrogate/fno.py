@@ -96,6 +96,11 @@ class FNO:
             ana = np.concatenate([ac, as_], axis=0)           # [2K][Z]
             self.bsyn = self._hi_lo(syn)
             self.ban = self._hi_lo(ana)
+            # the backward pass applies the transposes: the analysis matrix
+            # transposed as its synthesis ([Z][2K]) and the synthesis matrix
+            # transposed as its analysis ([2K][Z])
+            self.bsyn_t = self._hi_lo(ana.T)
+            self.ban_t = self._hi_lo(syn.T)
         self.step_count = 0
         self._bufs = {}
         self.refresh()
@@ -281,8 +286,12 @@ class FNO:
         k.mse(pred, 1, target.reshape(-1), mask.reshape(-1), dpred, R, 2.0 / max(cnt, 1.0), loss)
         return loss, dpred
 
-    def backward(self, dpred):
-        """Accumulate parameter gradients of the last train-mode forward."""
+    def backward(self, dpred, fused=None):
+        """Accumulate parameter gradients of the last train-mode forward;
+        through the fused backward layers when the network's shape has them
+        (fused=None or True), else (or with fused=False) layer by layer."""
+        if fused and not self.fusable:
+            raise ValueError("the fused backward kernels are built for Z=64, width=128, modes=16, features=128")
         cfg = self.cfg
         S, Z, F, Cw, Km, L = self._S, cfg.depth_points, cfg.features, cfg.width, cfg.modes, cfg.layers
         R = S * Z
@@ -297,6 +306,8 @@ class FNO:
         # y = q P2 + c2
         k.gemm(q, dpred, g["P2"], Cw, 1, R, ta=True, epi=k.EPI_ATOMIC, splits=k.splits_for(Cw, 1, R))
         k.colsum(dpred, g["c2"], R, 1)
+        if self.fusable if fused is None else fused:
+            return self._backward_fused(dpred, S)
         dh = self._buf("dh", (R, Cw), f32)
         k.gemm(dpred, self.bf["P2"], dh, R, Cw, 1, tb=True, epi=k.EPI_F32)
         dpre = self._buf("dpre", (R, Cw), bf)
@@ -327,6 +338,49 @@ class FNO:
         k.gelu_bwd(dh, None, dpre)
         k.gemm(self._x, dpre, g["W0"], F, Cw, R, ta=True, epi=k.EPI_ATOMIC, splits=k.splits_for(F, Cw, R))
         k.colsum(dpre, g["b0"], R, Cw)
+
+    def _backward_fused(self, dpred, S):
+        """The rest of the backward pass on the fused kernels: the projection's
+        GELU gradient in one pass, then per layer the weight-gradient products,
+        the mode-mixing gradient and one fused backward layer (pointwise
+        product with Wp^T, transposed analysis, GELU' of the layer below, bias
+        gradient partials, transposed synthesis for the layer below)."""
+        cfg = self.cfg
+        Z, Cw, Km, L = cfg.depth_points, cfg.width, cfg.modes, cfg.layers
+        R = S * Z
+        bf, f32 = torch.bfloat16, torch.float32
+        g, h = self.grads, self._h
+        part = self._buf("csum", (S, Cw), f32)
+        dpre = [self._buf("dpre", (R, Cw), bf), self._buf("dpre2", (R, Cw), bf)]
+        dmh = self._buf("dmh", (Km, S, 2 * Cw), bf)
+        dxh = self._buf("dxh", (Km, S, 2 * Cw), bf)
+        # q = GELU(h_L P1 + c1), y = q P2 + c2
+        k.outer_gelu_bwd(dpred, self.params["P2"].reshape(-1), self._buf("preq", (R, Cw), f32), dpre[0], part,
+                         S, Z, Cw)
+        k.colsum(part, g["c1"], S, Cw)
+        k.gemm(h[L], dpre[0], g["P1"], Cw, Cw, R, ta=True, epi=k.EPI_ATOMIC, splits=k.splits_for(Cw, Cw, R))
+        k.fno_layer_bwd(dpre[0], self.bf["P1"], dpre[1], part, S, ban=self.ban_t, dmh=dmh,
+                        pin=self._buf(f"pre{L}", (R, Cw), f32))
+        cur = 1
+        for l in range(L, 0, -1):
+            d = dpre[cur]
+            k.colsum(part, g[f"bp{l}"], S, Cw)
+            k.gemm(h[l - 1], d, g[f"Wp{l}"], Cw, Cw, R, ta=True, epi=k.EPI_ATOMIC, splits=k.splits_for(Cw, Cw, R))
+            xh = self._buf(f"xh{l}", (Km, S, 2 * Cw), bf)
+            k.gemm(xh, dmh, self.gblk[l], 2 * Cw, 2 * Cw, S, ta=True, batch=Km, sa=S * 2 * Cw, sb=S * 2 * Cw,
+                   sc=4 * Cw * Cw, epi=k.EPI_ATOMIC, splits=k.splits_for(2 * Cw, 2 * Cw, S, batch=Km))
+            k.gemm(dmh, self.blk[l], dxh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+                   sc=S * 2 * Cw, epi=k.EPI_BF16)
+            below = l > 1
+            k.fno_layer_bwd(d, self.bf[f"Wp{l}"], dpre[1 - cur], part, S, dxh=dxh, bsyn=self.bsyn_t,
+                            ban=self.ban_t if below else None, dmh=dmh if below else None,
+                            pin=self._buf(f"pre{l - 1}", (R, Cw), f32) if below else None)
+            cur = 1 - cur
+            k.spec_fold(self.gblk[l], g[f"Wr{l}"], g[f"Wi{l}"], Km, Cw)
+        # lift (no activation)
+        k.gemm(self._x, dpre[cur], g["W0"], cfg.features, Cw, R, ta=True, epi=k.EPI_ATOMIC,
+               splits=k.splits_for(cfg.features, Cw, R))
+        k.colsum(part, g["b0"], S, Cw)
 
     def step(self, lr, wd=0.0, b1=0.9, b2=0.999, eps=1e-8):
         """One Adam step on every parameter from the accumulated gradients."""

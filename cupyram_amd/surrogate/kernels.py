@@ -116,9 +116,22 @@ def spec_fold(g, gr, gi, K, Cw):
 
 
 def colsum(x, out, M, N):
-    _need(x, torch.bfloat16, M * N, "colsum x")
+    if x.dtype not in (torch.float32, torch.bfloat16):
+        raise ValueError("colsum: bf16 or f32 input")
+    _need(x, x.dtype, M * N, "colsum x")
     _need(out, torch.float32, N, "colsum out")
-    _abi.check(_abi.lib().cpr_fno_colsum(_p(x), _p(out), M, N, _stream()))
+    _abi.check(_abi.lib().cpr_fno_colsum(_p(x), int(x.dtype == torch.float32), _p(out), M, N, _stream()))
+
+
+def outer_gelu_bwd(dy, p, pre, dx, part, S, Z, Cw):
+    """dx = dy p^T * GELU'(pre) (bf16), part[s][c] = sum_z dx[s][z][c] (cpr_fno_outer_gelu_bwd)."""
+    _need(dy, torch.bfloat16, S * Z, "outer_gelu_bwd dy")
+    _need(p, torch.float32, Cw, "outer_gelu_bwd p")
+    _need(pre, torch.float32, S * Z * Cw, "outer_gelu_bwd pre")
+    _need(dx, torch.bfloat16, S * Z * Cw, "outer_gelu_bwd dx")
+    _need(part, torch.float32, S * Cw, "outer_gelu_bwd part")
+    _abi.check(_abi.lib().cpr_fno_outer_gelu_bwd(_p(dy), _p(p), _p(pre), _p(dx), _p(part), int(S), int(Z),
+                                                 int(Cw), _stream()))
 
 
 def mse(pred, ldp, target, mask, dpred, P, scale, loss_sum):
@@ -176,6 +189,29 @@ def fno_layer(hin, wt, bias, hout, S, *, mh=None, bsyn=None, ban=None, xh=None, 
         _need(pre, torch.float32, S * Z * Cw, "fno_layer pre")
     _abi.check(_abi.lib().cpr_fno_layer(int(syn), int(act), int(ana), _p(hin), _p(wt), _p(bias), _p(mh),
                                         _p(bsyn), _p(ban), _p(hout), _p(xh), _p(pre), int(S), _stream()))
+
+
+def fno_layer_bwd(din, w, dout, csum, S, *, dxh=None, bsyn=None, ban=None, dmh=None, pin=None):
+    """Fused backward layer (cpr_fno_layer_bwd): dout = (din W^T + syn(dxh)) *
+    GELU'(pin), csum = its sums over depth, dmh = its transposed synthesis
+    when ban/dmh are given."""
+    Z, Cw, K2 = FUSED_Z, FUSED_C, 2 * FUSED_K
+    _need(din, torch.bfloat16, S * Z * Cw, "fno_layer_bwd din")
+    _need(w, torch.bfloat16, Cw * Cw, "fno_layer_bwd w")
+    _need(dout, torch.bfloat16, S * Z * Cw, "fno_layer_bwd dout")
+    _need(csum, torch.float32, S * Cw, "fno_layer_bwd csum")
+    if dxh is None and dmh is None:
+        raise ValueError("fno_layer_bwd: give dxh, dmh or both")
+    if dxh is not None:
+        _need(dxh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer_bwd dxh")
+        _need(bsyn, torch.bfloat16, 2 * Z * K2, "fno_layer_bwd bsyn")
+    if dmh is not None:
+        _need(ban, torch.bfloat16, 2 * K2 * Z, "fno_layer_bwd ban")
+        _need(dmh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer_bwd dmh")
+    if pin is not None:
+        _need(pin, torch.float32, S * Z * Cw, "fno_layer_bwd pin")
+    _abi.check(_abi.lib().cpr_fno_layer_bwd(_p(din), _p(w), _p(dxh), _p(bsyn), _p(ban), _p(pin), _p(dout),
+                                            _p(dmh), _p(csum), int(S), _stream()))
 
 
 def fno_project(h, p1t, c1, p2, c2, y, S):

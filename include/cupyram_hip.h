@@ -195,8 +195,13 @@ int cpr_fno_gelu_bwd(const float *dy, const float *pre, void *dx, int64_t n, voi
  * the gradients of Wr and Wi. */
 int cpr_fno_spec_expand(const float *wr, const float *wi, void *blk, int K, int C, void *stream);
 int cpr_fno_spec_fold(const float *g, float *gr, float *gi, int K, int C, void *stream);
-/* out[n] += sum_m x[m][n], x bf16 [M][N]. */
-int cpr_fno_colsum(const void *x, float *out, int64_t M, int N, void *stream);
+/* out[n] += sum_m x[m][n], x [M][N] bf16 (or f32 if x_is_f32). */
+int cpr_fno_colsum(const void *x, int x_is_f32, float *out, int64_t M, int N, void *stream);
+/* Backward of y = GELU(pre) . p for a 1-column p [C] (f32): dx [S][Z][C]
+ * bf16 = dy[s][z] p[c] GELU'(pre[s][z][c]) with dy [S][Z] bf16, and
+ * part [S][C] f32 = the sums of dx over z (bias gradient partials). */
+int cpr_fno_outer_gelu_bwd(const void *dy, const float *p, const float *pre, void *dx, float *part, int S, int Z,
+                           int C, void *stream);
 /* Masked squared error on column 0 of pred [P][ldp]: dpred (bf16, same
  * layout) = scale*mask*(pred-target) in column 0 and 0 elsewhere;
  * loss_sum[0] += sum (mask*(pred-target))^2, loss_sum[1] += sum mask. */
@@ -220,6 +225,17 @@ int cpr_fno_fill(float *x, int64_t n, float v, void *stream);
  * layer. */
 int cpr_fno_layer(int syn, int act, int ana, const void *hin, const void *wt, const float *bias, const void *mh,
                   const void *bsyn, const void *ban, void *hout, void *xh, float *pre, int S, void *stream);
+/* Fused backward layer, same shapes as cpr_fno_layer: dout [S][64][128]
+ * bf16 = (din . W^T + syn(dxh)) * GELU'(pin), with w = W ([in][out] bf16,
+ * the layer's own layout), dxh [16][S][256] the mode-mixing gradients (or
+ * NULL: no spectral branch), bsyn the analysis matrix transposed ([2][64][32]
+ * hi, lo), pin the FP32 pre-activation of the layer below (NULL: no GELU,
+ * the lift); csum [S][128] f32 = the sums of dout over the 64 depths; and
+ * (dmh non-NULL) dmh [16][S][256] = the synthesis matrix's transpose applied
+ * to dout, with ban [2][32][64] the synthesis matrix (hi, lo).  At least one
+ * of dxh and dmh is given. */
+int cpr_fno_layer_bwd(const void *din, const void *w, const void *dxh, const void *bsyn, const void *ban,
+                      const float *pin, void *dout, void *dmh, float *csum, int S, void *stream);
 /* y [S*64] f32 = GELU(h . P1 + c1) . P2 + c2 with p1t = P1 transposed
  * ([out][in] bf16), p2 [128] f32. */
 int cpr_fno_project(const void *h, const void *p1t, const float *c1, const float *p2, float c2, float *y, int S,

@@ -192,8 +192,12 @@ def test_forward_matches_float64_reference(require_gpu):
         assert torch.isfinite(pre).all() and float(pre.abs().max()) > 0
 
 
-def test_gradients_match_float64_autograd(require_gpu):
-    cfg, x, tgt, mask = _small_batch(11)
+@pytest.mark.parametrize("fused,S", [(True, 6), (True, 7), (False, 6)])
+def test_gradients_match_float64_autograd(require_gpu, fused, S):
+    """Both backward paths (fused backward layers; transforms, products and
+    GELU' as separate kernels) against float64 autograd; S = 7 leaves the
+    second column of the fused kernels' last block empty."""
+    cfg, x, tgt, mask = _small_batch(11, S=S)
     fno = FNO(cfg, seed=2)
     # non-zero biases so that every path carries signal
     for n in fno.names:
@@ -202,7 +206,7 @@ def test_gradients_match_float64_autograd(require_gpu):
     fno.refresh()
     pred = fno.forward(x.cuda(), train=True)
     loss, dpred = fno.loss_grad(pred, tgt.cuda(), mask.cuda())
-    fno.backward(dpred)
+    fno.backward(dpred, fused=fused)
     torch.cuda.synchronize()
     ref_loss, _, ref_grads = R.loss_and_grads(R.reference_params(fno), x.double(), tgt.double(), mask.double(),
                                               cfg)
