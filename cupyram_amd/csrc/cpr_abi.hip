@@ -416,7 +416,7 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
     return bail(fail("upload failed"));
   if (p->variant == CPR_VARIANT_TEAM) {
     // per-term constants of the team march, formed once on the device
-    const int64_t nq = static_cast<int64_t>(cfg->nrays) * 3 * np_ * 6;
+    const int64_t nq = static_cast<int64_t>(cfg->nrays) * 3 * np_ * kTermStride;
     if (dalloc(p, &p->qtab_d, static_cast<size_t>(nq) * p->real_size())) return bail(-1);
     const MarchArgs m = p->args();
     const int block = 64, grid = (cfg->nrays + block - 1) / block;

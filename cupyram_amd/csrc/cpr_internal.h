@@ -33,6 +33,10 @@ struct RayState {
 };
 
 // Everything a march kernel needs; passed by value.
+// values per Pade term in the team march's constant table: q, 4q, rho / c,
+// renormalisation factor, rho (re, im each)
+constexpr int kTermStride = 10;
+
 struct MarchArgs {
   const cpr_ray *rays;
   const cpr_env *envs;
@@ -60,7 +64,7 @@ struct MarchArgs {
   void *dbuf;                // Galerkin coefficients of every team row (streamed
                              // by deep teams; saved between launches by the others)
   void *mbuf;                // FP64 water and bottom media of every team row
-  const void *qtab;          // team per-term constants [nrays][3 sets][np][q, 4q, rho] in R
+  const void *qtab;          // team per-term constants [nrays][3 sets][np][kTermStride] in R (k_team_terms)
   // layout of the field u: team_rows = 0: depth-major [row][ld]; > 0: the
   // team layout [ray][k][lane] of team_rows rows x team_lanes lanes
   int team_rows, team_lanes;

@@ -16,6 +16,11 @@
 //      parallel cyclic reduction across the team (wave shuffles when T <= 64,
 //      LDS otherwise);
 //   4. back-substitutes its rows.
+// In complex64 the factor's right-hand side is applied without a product
+// with D: with c = (1 - p1/p2) k0^2 / (6 p2) the factor equals
+// u' = (p1/p2) u + c y, (M + p2 D) y = (6/k0^2) M u, and the lanes march the
+// field divided by the running product of the c's (multiplied back at the
+// end of every step and wherever the product would leave [2^-30, 2^30]).
 // This solves the same linear system as the reference's top-to-bottom
 // Thomas sweep (fused_kernel.py:124-152) in a different elimination order.
 // The range update (updat.py:38-140), medium rebuild where the water/bottom
@@ -64,6 +69,24 @@ __device__ inline cx<R> cnmul(cx<R> a, cx<R> b) {
 template <class R>
 __device__ inline cx<R> cfma(cx<R> a, cx<R> b, cx<R> c) {
   return cxm<R>(a.re + b.re * c.re - b.im * c.im, a.im + b.re * c.im + b.im * c.re);
+}
+
+// Complex a + b c, a - b c and a b with every fused multiply-add written
+// out, so that two evaluations of one expression round alike wherever the
+// compiler places them: the halo rows a lane carries from factor to factor
+// (its neighbours' first and last rows) must equal, bit for bit, the rows
+// those neighbours compute, or a launch would not match two shorter ones.
+template <class R>
+__device__ inline cx<R> xfma(cx<R> a, cx<R> b, cx<R> c) {
+  return cxm<R>(fma(-b.im, c.im, fma(b.re, c.re, a.re)), fma(b.im, c.re, fma(b.re, c.im, a.im)));
+}
+template <class R>
+__device__ inline cx<R> xfms(cx<R> a, cx<R> b, cx<R> c) {
+  return cxm<R>(fma(b.im, c.im, fma(-b.re, c.re, a.re)), fma(-b.im, c.re, fma(-b.re, c.im, a.im)));
+}
+template <class R>
+__device__ inline cx<R> xmul(cx<R> a, cx<R> b) {
+  return cxm<R>(fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re));
 }
 
 // Copy a team-uniform value into scalar registers.
@@ -476,16 +499,24 @@ __device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
 }
 
 // Downward elimination with the spike through the lane's rows of one Pade
-// factor (q [1 4 1] + D) x = (q [1 4 1] + rho D) u.  REAL: the wavefront's
-// coefficients are real (every row's stencil lies in the water, where the
-// wavenumber is real), so the entries q + d and the product D u take real
-// operands: 9 fewer instructions per row, and streamed rows read half the
-// bytes.  Rows past nz (k >= mv) get a zero reciprocal pivot, so c', w and d'
-// vanish there and stay zero through the upward sweep and the back
-// substitution.
-template <bool REAL, class R, int M, bool DS>
+// factor (q [1 4 1] + D) x = (q [1 4 1] + rho D) u (f = rho), or, SC, in the
+// scaled form of k_team_terms: (q [1 4 1] + D) y' = [1 4 1] u' and
+// x' = y' + rc u' (f = rc = rho / c), whose right-hand side needs no product
+// with D.  REAL: the wavefront's coefficients are real (every row's stencil
+// lies in the water, where the wavenumber is real), so the entries q + d
+// take real operands and streamed rows read half the bytes.  Rows past nz
+// (k >= mv) get a zero reciprocal pivot, so c', w and d' vanish there and
+// stay zero through the upward sweep and the back substitution.
+//
+// SC: the lane keeps, per row, e_k = d'_k + rc (u'_k + c'_k u'_{k+1}) for
+// k < M - 2, e = d' + rc u' for k = M - 2 and d' for the last row: the upward
+// sweep D_k = e_k - c'_k D_{k+1} then yields D_k + rc u'_k (the back
+// substitution's x' without the stored u'), and the last row's u' is
+// returned in ulast for x'_{M-1} = y'_{M-1} + rc u'_{M-1}.
+template <bool REAL, bool SC, class R, int M, bool DS>
 __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coefs<R, M, DS> &C, cx<R> q,
-                                      cx<R> q4, cx<R> rho, cx<R> um, cx<R> u_next, int mv) {
+                                      cx<R> q4, cx<R> f, cx<R> um, cx<R> u_next, int mv, cx<R> &ulast) {
+  static_assert(M >= 2, "team lanes hold at least two rows");
   cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(0, 0), dprev = cxm<R>(0, 0);
   using Row = std::conditional_t<REAL, CoefHalf<R>, CoefRow<R>>;
   const auto V = C.view();
@@ -515,16 +546,21 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coe
       r1 = cxm<R>(q.re + cr.d1, q.im);
       r2 = cxm<R>(q4.re + cr.d2, q4.im);
       r3 = cxm<R>(q.re + cr.d3, q.im);
-      du = cxm<R>(cr.d1 * um.re + cr.d2 * uc.re + cr.d3 * up.re, cr.d1 * um.im + cr.d2 * uc.im + cr.d3 * up.im);
+      if constexpr (!SC)
+        du = cxm<R>(cr.d1 * um.re + cr.d2 * uc.re + cr.d3 * up.re, cr.d1 * um.im + cr.d2 * uc.im + cr.d3 * up.im);
     } else {
       r1 = q + cr.d1;
       r2 = q4 + cr.d2;
       r3 = q + cr.d3;
-      du = cfma(cfma(cr.d1 * um, cr.d2, uc), cr.d3, up);
+      if constexpr (!SC) du = cfma(cfma(cr.d1 * um, cr.d2, uc), cr.d3, up);
     }
     const cx<R> s = um + up;
     const cx<R> ts = cxm<R>(s.re + R(4) * uc.re, s.im + R(4) * uc.im);
-    const cx<R> rhs = cfma(q * ts, rho, du);
+    cx<R> rhs;
+    if constexpr (SC)
+      rhs = ts;
+    else
+      rhs = cfma(q * ts, f, du);  // f = rho
     const cx<R> den = k == 0 ? r2 : cfms(r2, r1, cprev);
     R m = rcp_real<R>(den.re * den.re + den.im * den.im);
     m = k < mv ? m : R(0);
@@ -534,7 +570,16 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coe
     const cx<R> wk = k == 0 ? r1 * inv : cnmul(r1 * wprev, inv);
     const cx<R> dk = k == 0 ? rhs * inv : cfms(rhs, r1, dprev) * inv;
     X.set(k, cc, wk);
-    L.u[k] = dk;
+    if constexpr (!SC)
+      L.u[k] = dk;
+    else if (k < M - 2)
+      L.u[k] = cfma(dk, f, cfma(uc, cc, up));  // f = rc
+    else if (k == M - 2)
+      L.u[k] = cfma(dk, f, uc);
+    else {
+      L.u[k] = dk;
+      ulast = uc;
+    }
     cprev = cc;
     wprev = wk;
     dprev = dk;
@@ -606,19 +651,28 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
     // every row stencil of this wavefront above the seabed row: real coefficients
     const bool wreal = __builtin_amdgcn_ballot_w64(g0 + M > st.iz) == 0;
     // Pade factor (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1],
-    // divided by p2: (q [1 4 1] + D) x = (q [1 4 1] + rho D) u with
-    // q = k0^2 / (6 p2), rho = p1 / p2, formed once per plan in FP64
-    // (k_team_terms) and read here through scalar loads.
-    const R *qt = static_cast<const R *>(a.qtab) + (static_cast<int64_t>(b) * 3 + st.pset) * P.np * 6;
+    // divided by p2: (q [1 4 1] + D) x = (q [1 4 1] + rho D) u.  complex64
+    // applies it as x = rho u + c y with (q [1 4 1] + D) y = [1 4 1] u,
+    // c = (1 - rho) q, on the field scaled by the running product of the c's
+    // (x' = y' + rc u', rc = rho / c: 12 fewer VALU instructions per complex
+    // row and 6 per real one); complex128 keeps the direct form, whose fewer
+    // live values across the reduced system spill less (the scaled form
+    // measured 215.8 instead of 196.9 ms per 100-step Munk launch).  The
+    // constants and the renormalisation factors rn are formed once per plan
+    // in FP64 (k_team_terms) and read here through scalar loads.
+    constexpr bool SC = std::is_same<R, float>::value;
+    const R *qt = static_cast<const R *>(a.qtab) + (static_cast<int64_t>(b) * 3 + st.pset) * P.np * kTermStride;
     for (int j = 0; j < P.np; ++j) {
-      const cx<R> q = cxm<R>(qt[6 * j], qt[6 * j + 1]);
-      const cx<R> q4 = cxm<R>(qt[6 * j + 2], qt[6 * j + 3]);
-      const cx<R> rho = cxm<R>(qt[6 * j + 4], qt[6 * j + 5]);
+      const R *qj = qt + kTermStride * j;
+      const cx<R> q = cxm<R>(qj[0], qj[1]);
+      const cx<R> q4 = cxm<R>(qj[2], qj[3]);
+      const cx<R> rc = SC ? cxm<R>(qj[4], qj[5]) : cxm<R>(qj[8], qj[9]);  // rc, or rho
       // 1. downward elimination with the spike
+      cx<R> ulast = cxm<R>(0, 0);
       if (wreal)
-        eliminate_down<true>(L, X, C, q, q4, rho, um, u_next, mv);
+        eliminate_down<true, SC>(L, X, C, q, q4, rc, um, u_next, mv, ulast);
       else
-        eliminate_down<false>(L, X, C, q, q4, rho, um, u_next, mv);
+        eliminate_down<false, SC>(L, X, C, q, q4, rc, um, u_next, mv, ulast);
       // 2. upward sweep: x_k = D_k - W_k y_{t-1} - C_k y_t for k < M-1
 #pragma unroll
       for (int k = M - 3; k >= 0; --k) {
@@ -626,9 +680,11 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
         L.u[k] = cfms(L.u[k], ck, L.u[k + 1]);
         X.set(k, cnmul(ck, X.c(k + 1)), cfms(X.w(k), ck, X.w(k + 1)));
       }
-      // 3. reduced equation a y_{t-1} + b y_t + c y_{t+1} = f, then PCR
-      cx<R> nD, nW, nC;
-      team.next3(L.u[0], X.w(0), X.c(0), nD, nW, nC);
+      // 3. reduced equation a y_{t-1} + b y_t + c y_{t+1} = f, then PCR.  The
+      // next lane's D_0 carries rc u'_0 (its first row's field, u_next here).
+      cx<R> nDx, nW, nC;
+      team.next3(L.u[0], X.w(0), X.c(0), nDx, nW, nC);
+      const cx<R> nD = SC ? cfms(nDx, rc, u_next) : nDx;
       const cx<R> ce = X.c(M - 1);
       cx<R> ea = t > 0 ? X.w(M - 1) : cxm<R>(0, 0);
       const cx<R> eb = cfms(cxm<R>(1, 0), ce, nW);
@@ -663,10 +719,25 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
       cx<R> yl, yr;
       team.neighbours(yc, yl, yr);
 #pragma unroll
-      for (int k = 0; k < M - 1; ++k) L.u[k] = cfms(cfms(L.u[k], X.w(k), yl), X.c(k), yc);
-      L.u[M - 1] = yc;
-      um = yl;
-      u_next = cfms(cfms(nD, nW, yc), nC, yr);
+      for (int k = 0; k < M - 1; ++k) L.u[k] = xfms(xfms(L.u[k], X.w(k), yl), X.c(k), yc);
+      if constexpr (SC) {
+        L.u[M - 1] = xfma(yc, rc, ulast);
+        um = xfma(yl, rc, um);
+      } else {
+        L.u[M - 1] = yc;
+        um = yl;
+      }
+      u_next = xfms(xfms(nDx, nW, yc), nC, yr);
+      if constexpr (SC) {
+        // 5. multiply the scale back in after the marked terms (always the last)
+        const cx<R> rn = cxm<R>(qj[6], qj[7]);
+        if (rn.re != R(0) || rn.im != R(0)) {
+#pragma unroll
+          for (int k = 0; k < M; ++k) L.u[k] = xmul(L.u[k], rn);
+          um = xmul(um, rn);
+          u_next = xmul(u_next, rn);
+        }
+      }
     }
     if (a.cells) {
       const int bin = st.tlc + 1;
@@ -760,30 +831,77 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
 }  // namespace
 
 // Per-term constants of the team march for every ray, Pade set and term,
-// formed in FP64 and rounded to the march precision: the factor
-// (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1] divided by p2 is
-// (q [1 4 1] + D) x = (q [1 4 1] + rho D) u, q = k0^2 / (6 p2), rho = p1 / p2;
-// stored as q, 4 q, rho (re, im each).
+// formed in FP64 and rounded to the march precision.  The factor
+// (A + p2 D) x = (A + p1 D) u with A = (k0^2/6) [1 4 1], divided by p2, is
+// (q [1 4 1] + D) x = (q [1 4 1] + rho D) u with q = k0^2 / (6 p2) and
+// rho = p1 / p2; its right-hand side matrix is rho (q [1 4 1] + D) +
+// (1 - rho) q [1 4 1], so x = rho u + c y with c = (1 - rho) q and
+// (q [1 4 1] + D) y = [1 4 1] u.  The march carries the field divided by the
+// running product of the c's, x' = y' + (rho / c) u', and multiplies it back
+// after the terms marked here (rn != 0: the product since the last such
+// term), always after the last one and otherwise before the product would
+// leave [2^-30, 2^30].  Stored per term: q, 4 q, rho / c, rn, rho (re, im
+// each); complex128 uses q, 4 q and rho (the direct form).
 template <class R>
 __global__ void k_team_terms(MarchArgs a, R *qtab) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= a.nrays) return;
   const cpr_ray P = a.rays[b];
+  const double a1d = P.k0 * P.k0 / 6.0;
   for (int set = 0; set < 3; ++set) {
     const double *pd = a.pade + (static_cast<int64_t>(b) * 3 + set) * 2 * P.np * 2;
-    R *o = qtab + (static_cast<int64_t>(b) * 3 + set) * P.np * 6;
-    for (int j = 0; j < P.np; ++j) {
+    R *o = qtab + (static_cast<int64_t>(b) * 3 + set) * P.np * kTermStride;
+    // q, rho and c of term j
+    auto term = [&](int j, double &qr, double &qi, double &rr, double &ri, double &cr, double &ci) {
       const double p1r = pd[2 * j], p1i = pd[2 * j + 1];
       const double p2r = pd[2 * (P.np + j)], p2i = pd[2 * (P.np + j) + 1];
       const double n2 = 1.0 / (p2r * p2r + p2i * p2i);
       const double ir = p2r * n2, ii = -p2i * n2;
-      const double a1d = P.k0 * P.k0 / 6.0;
-      o[6 * j + 0] = static_cast<R>(a1d * ir);
-      o[6 * j + 1] = static_cast<R>(a1d * ii);
-      o[6 * j + 2] = static_cast<R>(4.0 * a1d * ir);
-      o[6 * j + 3] = static_cast<R>(4.0 * a1d * ii);
-      o[6 * j + 4] = static_cast<R>(p1r * ir - p1i * ii);
-      o[6 * j + 5] = static_cast<R>(p1r * ii + p1i * ir);
+      qr = a1d * ir;
+      qi = a1d * ii;
+      rr = p1r * ir - p1i * ii;
+      ri = p1r * ii + p1i * ir;
+      // a factor with p1 = p2 is the identity (c = 0): a c of 1e-12 q
+      // changes x by 1e-12 of the correction term and keeps rho / c finite
+      double er = 1.0 - rr, ei = -ri;
+      if (er * er + ei * ei < 1e-24) {
+        er = 1e-12;
+        ei = 0.0;
+      }
+      cr = er * qr - ei * qi;
+      ci = er * qi + ei * qr;
+    };
+    const double lo2 = 0x1p-60, hi2 = 0x1p60;  // squared bounds of |product|
+    double sr = 1.0, si = 0.0;                  // product of c since the last renormalisation
+    for (int j = 0; j < P.np; ++j) {
+      double qr, qi, rr, ri, cr, ci;
+      term(j, qr, qi, rr, ri, cr, ci);
+      const double nc = 1.0 / (cr * cr + ci * ci);
+      o[kTermStride * j + 0] = static_cast<R>(qr);
+      o[kTermStride * j + 1] = static_cast<R>(qi);
+      o[kTermStride * j + 2] = static_cast<R>(4.0 * qr);
+      o[kTermStride * j + 3] = static_cast<R>(4.0 * qi);
+      o[kTermStride * j + 4] = static_cast<R>((rr * cr + ri * ci) * nc);  // rho / c
+      o[kTermStride * j + 5] = static_cast<R>((ri * cr - rr * ci) * nc);
+      const double tr = sr * cr - si * ci;
+      si = sr * ci + si * cr;
+      sr = tr;
+      bool renorm = j == P.np - 1;
+      if (!renorm) {
+        double q2r, q2i, r2r, r2i, c2r, c2i;
+        term(j + 1, q2r, q2i, r2r, r2i, c2r, c2i);
+        const double xr = sr * c2r - si * c2i, xi = sr * c2i + si * c2r;
+        const double m2 = xr * xr + xi * xi;
+        renorm = !(m2 >= lo2 && m2 <= hi2);
+      }
+      o[kTermStride * j + 6] = static_cast<R>(renorm ? sr : 0.0);
+      o[kTermStride * j + 7] = static_cast<R>(renorm ? si : 0.0);
+      o[kTermStride * j + 8] = static_cast<R>(rr);
+      o[kTermStride * j + 9] = static_cast<R>(ri);
+      if (renorm) {
+        sr = 1.0;
+        si = 0.0;
+      }
     }
   }
 }
