@@ -13,8 +13,9 @@
 //      unknown y_t: x_k = D_k - W_k y_{t-1} - C_k y_t;
 //   3. its last row plus the next lane's first row give one tridiagonal
 //      equation in (y_{t-1}, y_t, y_{t+1}); the T equations are solved by
-//      parallel cyclic reduction across the team (wave shuffles when T <= 64,
-//      LDS otherwise);
+//      parallel cyclic reduction with wave shuffles (T = 64), or, for larger
+//      teams, by cyclic reduction through LDS down to 64 unknowns, PCR in one
+//      wavefront and the reduction undone;
 //   4. back-substitutes its rows.
 // In complex64 the factor's right-hand side is applied without a product
 // with D: with c = (1 - p1/p2) k0^2 / (6 p2) the factor equals
@@ -117,11 +118,8 @@ __device__ inline R shfl_down_r(R v, int s, int w) {
 // follows its last reads of it.  A buffer is an array of per-lane slots of
 // three complex values (one exchange writes one slot per lane, so reading a
 // neighbour's values takes one address), with zeroed margins of kPad slots on
-// either side: reads at distance s <= kPad need no bounds test, and at the
-// wider PCR distances (s = 64, 128, .. with T a multiple of 64) whether lane
-// t +- s lies in the team is the same for every lane of a wavefront, a
-// scalar branch.  The margins are zeroed once per launch (init) and never
-// written.
+// either side, so that reads of lanes t - 1 and t + 1 need no bounds test.
+// The margins are zeroed once per launch (init) and never written.
 constexpr int kPad = 32;
 
 template <class R>
@@ -129,10 +127,44 @@ struct Slot {
   cx<R> a, c, f;
 };
 
-// LDS bytes of an LDS-mode team of T lanes
+// Reduced system of an LDS-mode team (T > 64 lanes, a multiple of 64):
+// cyclic reduction levels k = 0 .. K-1 hold n_k = T / 2^k > 64 unknowns (all
+// even counts); level 0's equations go through the next exchange buffer and
+// those of levels 1 .. K-1 through the buffer the previous exchange used.
+// The solutions of levels 1 .. K need their own LDS arrays, one zero entry
+// past the end of each; their complex entries:
+__host__ __device__ constexpr int cr_values(int T) {
+  int c = 0;
+  for (int n = T; n > 64;) {
+    n = (n + 1) / 2;
+    c += n + 1;
+  }
+  return c;
+}
+
+// LDS bytes of an LDS-mode team of T lanes: the two exchange buffers, then
+// the reduced system's solution arrays
 template <class R>
 constexpr size_t team_lds_bytes(int T) {
-  return 2 * static_cast<size_t>(T + 2 * kPad) * sizeof(Slot<R>);
+  return 2 * static_cast<size_t>(T + 2 * kPad) * sizeof(Slot<R>) + cr_values(T) * sizeof(cx<R>);
+}
+
+// One elimination step of the reduced system, shared by PCR and cyclic
+// reduction: equation (a, 1, c | f) of an unknown with the equations of its
+// neighbours at distance s, (am, 1, cm | fm) before and (ap, 1, cp | fp)
+// after, gives the equation in the unknowns at distance 2s, normalised to a
+// unit diagonal.
+template <class R>
+__device__ inline void reduce_step(cx<R> &a, cx<R> &c, cx<R> &f, cx<R> am, cx<R> cm, cx<R> fm, cx<R> ap, cx<R> cp,
+                                   cx<R> fp) {
+  const cx<R> bn = cfms(cfms(cxm<R>(1, 0), a, cm), c, ap);
+  const cx<R> an = cnmul(a, am);
+  const cx<R> cn = cnmul(c, cp);
+  const cx<R> fn = cfms(cfms(f, a, fm), c, fp);
+  const cx<R> ib = fast_rcp(bn);
+  a = an * ib;
+  c = cn * ib;
+  f = fn * ib;
 }
 
 template <class R, int TW>
@@ -141,13 +173,117 @@ struct Team {
   Slot<R> *lds;  // LDS mode: 2 buffers of (kPad + T + kPad) slots
   int ph;
 
-  // zero both buffers (margins included); one barrier
+  // zero both buffers (margins included) and the reduced system's solution
+  // arrays; one barrier
   __device__ void init() {
     if constexpr (TW == 0) {
       const int n = 2 * (T + 2 * kPad);
       for (int i = t; i < n; i += T) lds[i] = Slot<R>{cxm<R>(0, 0), cxm<R>(0, 0), cxm<R>(0, 0)};
+      cx<R> *u = reinterpret_cast<cx<R> *>(lds + n);
+      for (int i = t; i < cr_values(T); i += T) u[i] = cxm<R>(0, 0);
       __syncthreads();
     }
+  }
+
+  // LDS mode: solve a_t y_{t-1} + y_t + c_t y_{t+1} = f_t over the team
+  // (a_0 = c_{T-1} = 0) and return y_t.  Cyclic reduction halves the system
+  // (the unknowns at even positions stay, held compacted by the lanes
+  // t < n/2, so whole wavefronts drop out) until 64 unknowns remain;
+  // wavefront 0 solves those by PCR with shuffles; the levels are then undone
+  // (even positions copy, odd ones solve their own equation).  Against PCR
+  // over all T lanes: 2K barriers instead of log2 T and a fraction of the
+  // VALU work (K = log2 (T / 64) levels).  Called right after an exchange
+  // (next3), whose buffer every lane has read before the first barrier here.
+  __device__ cx<R> solve_cr(cx<R> a, cx<R> c, cx<R> f, cx<R> &yl, cx<R> &yr) {
+    static_assert(TW == 0, "cyclic reduction runs on LDS teams");
+    Slot<R> *const L0 = next_buf();                     // level 0
+    Slot<R> *const L1 = lds + ph * (T + 2 * kPad) + kPad;  // levels 1 .. K-1, back to back
+    cx<R> *const U1 = reinterpret_cast<cx<R> *>(lds + 2 * (T + 2 * kPad));
+    const cx<R> z = cxm<R>(0, 0);
+    Slot<R> *L = L0;
+    int n = T;
+    L[t] = Slot<R>{a, c, f};
+    while (n > 64) {
+      __syncthreads();
+      const int n2 = n >> 1;
+      Slot<R> *const N = L == L0 ? L1 : L + n;
+      if (t < n2) {
+        // unknown 2t with its neighbours 2t - 1 (none for t = 0) and 2t + 1
+        const Slot<R> o = L[2 * t], p = L[2 * t + 1];
+        Slot<R> m = L[t > 0 ? 2 * t - 1 : 0];
+        if (t == 0) m = Slot<R>{z, z, z};
+        a = o.a;
+        c = o.c;
+        f = o.f;
+        reduce_step(a, c, f, m.a, m.c, m.f, p.a, p.c, p.f);
+        if (n2 > 64) N[t] = Slot<R>{a, c, f};
+      }
+      L = N;
+      n = n2;
+    }
+    cx<R> y = z;
+    if (t < 64) {
+      // lanes past the last level (T = 448: 56 unknowns) hold y = 0
+      if (t >= n) a = c = f = z;
+#pragma unroll
+      for (int l = 0; l < 6; ++l) {
+        const int s = 1 << l;
+        const bool lo = t >= s, hi = t + s < 64;
+        const cx<R> a1 = cxm<R>(shfl_up_r(a.re, s, 64), shfl_up_r(a.im, s, 64));
+        const cx<R> c1 = cxm<R>(shfl_up_r(c.re, s, 64), shfl_up_r(c.im, s, 64));
+        const cx<R> f1 = cxm<R>(shfl_up_r(f.re, s, 64), shfl_up_r(f.im, s, 64));
+        const cx<R> a2 = cxm<R>(shfl_down_r(a.re, s, 64), shfl_down_r(a.im, s, 64));
+        const cx<R> c2 = cxm<R>(shfl_down_r(c.re, s, 64), shfl_down_r(c.im, s, 64));
+        const cx<R> f2 = cxm<R>(shfl_down_r(f.re, s, 64), shfl_down_r(f.im, s, 64));
+        reduce_step(a, c, f, lo ? a1 : z, lo ? c1 : z, lo ? f1 : z, hi ? a2 : z, hi ? c2 : z, hi ? f2 : z);
+      }
+      y = f;
+    }
+    // undo the levels: level k's equations at L_k, level k+1's solution
+    // through U_{k+1} (n_{k+1} entries and a zero one); level 0 separately
+    int K = 0;
+    for (int m = T; m > 64; m >>= 1) ++K;
+    for (int k = K - 1; k >= 1; --k) {
+      Slot<R> *Lk = L0;
+      cx<R> *U = U1;
+      int nk = T;
+      for (int i = 0; i < k; ++i) {
+        Lk = i == 0 ? L1 : Lk + nk;
+        nk >>= 1;
+        U += nk + 1;
+      }
+      const int nk1 = nk >> 1;
+      if (t < nk1) U[t] = y;
+      __syncthreads();
+      if (t < nk) {
+        if ((t & 1) == 0) {
+          y = U[t >> 1];
+        } else {
+          const Slot<R> e = Lk[t];
+          y = xfms(xfms(e.f, e.a, U[(t - 1) >> 1]), e.c, U[(t + 1) >> 1]);
+        }
+      }
+    }
+    // level 0: every lane forms y_t and its neighbours' y_{t-1}, y_{t+1}
+    // (zero past the team's ends) by the same expressions their owners use,
+    // so no exchange follows
+    if (t < T / 2) U1[t] = y;
+    __syncthreads();
+    const int h = t >> 1;
+    if (t & 1) {
+      const Slot<R> e = L0[t];
+      const cx<R> u0 = U1[h], u1 = U1[h + 1];  // U1[T/2] is the zero entry
+      y = xfms(xfms(e.f, e.a, u0), e.c, u1);
+      yl = u0;
+      yr = u1;
+    } else {
+      const Slot<R> el = L0[t - 1], er = L0[t + 1];  // L0[-1]: the zero margin
+      const cx<R> um = t > 0 ? U1[h - 1] : z, u0 = U1[h], up = U1[h + 1];
+      y = u0;
+      yl = xfms(xfms(el.f, el.a, um), el.c, u0);
+      yr = xfms(xfms(er.f, er.a, u0), er.c, up);
+    }
+    return y;
   }
   // lane 0's slot of the next buffer
   __device__ Slot<R> *next_buf() {
@@ -207,50 +343,24 @@ struct Team {
       nc = n.f;
     }
   }
-  // one PCR exchange: (a, c, f) of lanes t-s and t+s
+  // one PCR exchange of a wavefront team: (a, c, f) of lanes t-s and t+s
   __device__ void pcr(int s, cx<R> a, cx<R> c, cx<R> f, cx<R> &am, cx<R> &cm, cx<R> &fm, cx<R> &ap, cx<R> &cp,
                       cx<R> &fp) {
-    if constexpr (TW > 0) {
-      const bool lo = t >= s, hi = t + s < T;
-      const cx<R> z = cxm<R>(0, 0);
-      const cx<R> a1 = cxm<R>(shfl_up_r(a.re, s, TW), shfl_up_r(a.im, s, TW));
-      const cx<R> c1 = cxm<R>(shfl_up_r(c.re, s, TW), shfl_up_r(c.im, s, TW));
-      const cx<R> f1 = cxm<R>(shfl_up_r(f.re, s, TW), shfl_up_r(f.im, s, TW));
-      const cx<R> a2 = cxm<R>(shfl_down_r(a.re, s, TW), shfl_down_r(a.im, s, TW));
-      const cx<R> c2 = cxm<R>(shfl_down_r(c.re, s, TW), shfl_down_r(c.im, s, TW));
-      const cx<R> f2 = cxm<R>(shfl_down_r(f.re, s, TW), shfl_down_r(f.im, s, TW));
-      am = lo ? a1 : z;
-      cm = lo ? c1 : z;
-      fm = lo ? f1 : z;
-      ap = hi ? a2 : z;
-      cp = hi ? c2 : z;
-      fp = hi ? f2 : z;
-    } else {
-      Slot<R> *B = next_buf();
-      B[t] = Slot<R>{a, c, f};
-      __syncthreads();
-      Slot<R> m, q;
-      if (s <= kPad) {
-        m = B[t - s];  // the margins hold zeros past the ends
-        q = B[t + s];
-      } else {
-        const Slot<R> z{cxm<R>(0, 0), cxm<R>(0, 0), cxm<R>(0, 0)};
-        if (__builtin_amdgcn_readfirstlane(t >= s))
-          m = B[t - s];
-        else
-          m = z;
-        if (__builtin_amdgcn_readfirstlane(t + s < T))
-          q = B[t + s];
-        else
-          q = z;
-      }
-      am = m.a;
-      cm = m.c;
-      fm = m.f;
-      ap = q.a;
-      cp = q.c;
-      fp = q.f;
-    }
+    static_assert(TW > 0, "LDS teams solve the reduced system with solve_cr");
+    const bool lo = t >= s, hi = t + s < T;
+    const cx<R> z = cxm<R>(0, 0);
+    const cx<R> a1 = cxm<R>(shfl_up_r(a.re, s, TW), shfl_up_r(a.im, s, TW));
+    const cx<R> c1 = cxm<R>(shfl_up_r(c.re, s, TW), shfl_up_r(c.im, s, TW));
+    const cx<R> f1 = cxm<R>(shfl_up_r(f.re, s, TW), shfl_up_r(f.im, s, TW));
+    const cx<R> a2 = cxm<R>(shfl_down_r(a.re, s, TW), shfl_down_r(a.im, s, TW));
+    const cx<R> c2 = cxm<R>(shfl_down_r(c.re, s, TW), shfl_down_r(c.im, s, TW));
+    const cx<R> f2 = cxm<R>(shfl_down_r(f.re, s, TW), shfl_down_r(f.im, s, TW));
+    am = lo ? a1 : z;
+    cm = lo ? c1 : z;
+    fm = lo ? f1 : z;
+    ap = hi ? a2 : z;
+    cp = hi ? c2 : z;
+    fp = hi ? f2 : z;
   }
 };
 
@@ -696,28 +806,26 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
         ec = ec * ib;
         ef = ef * ib;
       }
-      // unrolled so that each level's distance s is a constant (T <= 512)
+      cx<R> yc, yl, yr;
+      if constexpr (TW > 0) {
+        // PCR over the 64-lane wavefront, unrolled so that each level's
+        // distance s is a constant
 #pragma unroll
-      for (int l = 0; l < 9; ++l) {
-        const int s = 1 << l;
-        if (s >= T) break;
-        cx<R> am, cm, fm, ap, cp, fp;
-        team.pcr(s, ea, ec, ef, am, cm, fm, ap, cp, fp);
-        const cx<R> bn = cfms(cfms(cxm<R>(1, 0), ea, cm), ec, ap);
-        const cx<R> an = cnmul(ea, am);
-        const cx<R> cn = cnmul(ec, cp);
-        const cx<R> fn = cfms(cfms(ef, ea, fm), ec, fp);
-        const cx<R> ib = fast_rcp(bn);
-        ea = an * ib;
-        ec = cn * ib;
-        ef = fn * ib;
+        for (int l = 0; l < 6; ++l) {
+          const int s = 1 << l;
+          if (s >= T) break;
+          cx<R> am, cm, fm, ap, cp, fp;
+          team.pcr(s, ea, ec, ef, am, cm, fm, ap, cp, fp);
+          reduce_step(ea, ec, ef, am, cm, fm, ap, cp, fp);
+        }
+        yc = ef;
+        team.neighbours(yc, yl, yr);
+      } else {
+        yc = team.solve_cr(ea, ec, ef, yl, yr);
       }
       // 4. back substitution (rows past nz come out zero).  The next lane's
       // new first row is formed here from its (D, W, C) of step 3 exactly as
       // that lane forms it, so the next factor needs no halo exchange.
-      const cx<R> yc = ef;
-      cx<R> yl, yr;
-      team.neighbours(yc, yl, yr);
 #pragma unroll
       for (int k = 0; k < M - 1; ++k) L.u[k] = xfms(xfms(L.u[k], X.w(k), yl), X.c(k), yc);
       if constexpr (SC) {
