@@ -31,8 +31,10 @@ def _hipcc():
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
+    """Compile every source (in parallel, one compiler process per source)
+    and link the shared library."""
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    objs, procs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(HERE, "_build", src + ".o")
@@ -42,8 +44,11 @@ def build(verbose: bool = False) -> str:
             cmd = [_hipcc(), *HIP_FLAGS, "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"compilation failed: {', '.join(failed)}")
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     if verbose:

@@ -142,11 +142,13 @@ __host__ __device__ constexpr int cr_values(int T) {
   return c;
 }
 
-// LDS bytes of an LDS-mode team of T lanes: the two exchange buffers, then
-// the reduced system's solution arrays
+// LDS bytes of an LDS-mode team of T lanes: the two exchange buffers, the
+// reduced system's solution arrays, then the in-wave PCR buffer (64 slots
+// with zero margins)
 template <class R>
 constexpr size_t team_lds_bytes(int T) {
-  return 2 * static_cast<size_t>(T + 2 * kPad) * sizeof(Slot<R>) + cr_values(T) * sizeof(cx<R>);
+  return 2 * static_cast<size_t>(T + 2 * kPad) * sizeof(Slot<R>) + cr_values(T) * sizeof(cx<R>) +
+         static_cast<size_t>(64 + 2 * kPad) * sizeof(Slot<R>);
 }
 
 // One elimination step of the reduced system, shared by PCR and cyclic
@@ -170,17 +172,23 @@ __device__ inline void reduce_step(cx<R> &a, cx<R> &c, cx<R> &f, cx<R> am, cx<R>
 template <class R, int TW>
 struct Team {
   int t, T;
-  Slot<R> *lds;  // LDS mode: 2 buffers of (kPad + T + kPad) slots
+  Slot<R> *lds;  // LDS mode: 2 buffers of (kPad + T + kPad) slots; complex64 wavefront teams: lane 0's slot of the PCR buffer
   int ph;
 
   // zero both buffers (margins included) and the reduced system's solution
   // arrays; one barrier
   __device__ void init() {
+    if constexpr (TW > 0 && sizeof(R) == 4) {
+      // a wavefront team's own PCR buffer (lds: lane 0's slot): zeroed once
+      for (int i = t; i < TW + 2 * kPad; i += TW) lds[i - kPad] = Slot<R>{cxm<R>(0, 0), cxm<R>(0, 0), cxm<R>(0, 0)};
+    }
     if constexpr (TW == 0) {
       const int n = 2 * (T + 2 * kPad);
       for (int i = t; i < n; i += T) lds[i] = Slot<R>{cxm<R>(0, 0), cxm<R>(0, 0), cxm<R>(0, 0)};
       cx<R> *u = reinterpret_cast<cx<R> *>(lds + n);
       for (int i = t; i < cr_values(T); i += T) u[i] = cxm<R>(0, 0);
+      Slot<R> *pb = pcr_buf() - kPad;
+      for (int i = t; i < 64 + 2 * kPad; i += T) pb[i] = Slot<R>{cxm<R>(0, 0), cxm<R>(0, 0), cxm<R>(0, 0)};
       __syncthreads();
     }
   }
@@ -189,7 +197,8 @@ struct Team {
   // (a_0 = c_{T-1} = 0) and return y_t.  Cyclic reduction halves the system
   // (the unknowns at even positions stay, held compacted by the lanes
   // t < n/2, so whole wavefronts drop out) until 64 unknowns remain;
-  // wavefront 0 solves those by PCR with shuffles; the levels are then undone
+  // wavefront 0 solves those by PCR (through LDS in complex64, with shuffles in
+  // complex128); the levels are then undone
   // (even positions copy, odd ones solve their own equation).  Against PCR
   // over all T lanes: 2K barriers instead of log2 T and a fraction of the
   // VALU work (K = log2 (T / 64) levels).  Called right after an exchange
@@ -225,17 +234,38 @@ struct Team {
     if (t < 64) {
       // lanes past the last level (T = 448: 56 unknowns) hold y = 0
       if (t >= n) a = c = f = z;
+      if constexpr (sizeof(R) == 4) {
+        // complex64: PCR through an LDS buffer of the wavefront (81.2
+        // instead of 85.4 ms per 100-step Munk launch against shuffles, which
+        // move one dword per instruction).  A wavefront's LDS operations
+        // execute in order, so a lane reads its neighbours' slots right after
+        // the write of its own; the empty asm statements keep the compiler
+        // from moving the accesses across one another.  The margins hold
+        // zeros, the values past the team's ends.
+        Slot<R> *const PB = pcr_buf();
 #pragma unroll
-      for (int l = 0; l < 6; ++l) {
-        const int s = 1 << l;
-        const bool lo = t >= s, hi = t + s < 64;
-        const cx<R> a1 = cxm<R>(shfl_up_r(a.re, s, 64), shfl_up_r(a.im, s, 64));
-        const cx<R> c1 = cxm<R>(shfl_up_r(c.re, s, 64), shfl_up_r(c.im, s, 64));
-        const cx<R> f1 = cxm<R>(shfl_up_r(f.re, s, 64), shfl_up_r(f.im, s, 64));
-        const cx<R> a2 = cxm<R>(shfl_down_r(a.re, s, 64), shfl_down_r(a.im, s, 64));
-        const cx<R> c2 = cxm<R>(shfl_down_r(c.re, s, 64), shfl_down_r(c.im, s, 64));
-        const cx<R> f2 = cxm<R>(shfl_down_r(f.re, s, 64), shfl_down_r(f.im, s, 64));
-        reduce_step(a, c, f, lo ? a1 : z, lo ? c1 : z, lo ? f1 : z, hi ? a2 : z, hi ? c2 : z, hi ? f2 : z);
+        for (int l = 0; l < 6; ++l) {
+          const int s = 1 << l;
+          PB[t] = Slot<R>{a, c, f};
+          asm volatile("" ::: "memory");
+          const Slot<R> m = PB[t - s], p = PB[t + s];
+          asm volatile("" ::: "memory");
+          reduce_step(a, c, f, m.a, m.c, m.f, p.a, p.c, p.f);
+        }
+      } else {
+        // complex128: shuffles (the 48-byte LDS slots measured slower)
+#pragma unroll
+        for (int l = 0; l < 6; ++l) {
+          const int s = 1 << l;
+          const bool lo = t >= s, hi = t + s < 64;
+          const cx<R> a1 = cxm<R>(shfl_up_r(a.re, s, 64), shfl_up_r(a.im, s, 64));
+          const cx<R> c1 = cxm<R>(shfl_up_r(c.re, s, 64), shfl_up_r(c.im, s, 64));
+          const cx<R> f1 = cxm<R>(shfl_up_r(f.re, s, 64), shfl_up_r(f.im, s, 64));
+          const cx<R> a2 = cxm<R>(shfl_down_r(a.re, s, 64), shfl_down_r(a.im, s, 64));
+          const cx<R> c2 = cxm<R>(shfl_down_r(c.re, s, 64), shfl_down_r(c.im, s, 64));
+          const cx<R> f2 = cxm<R>(shfl_down_r(f.re, s, 64), shfl_down_r(f.im, s, 64));
+          reduce_step(a, c, f, lo ? a1 : z, lo ? c1 : z, lo ? f1 : z, hi ? a2 : z, hi ? c2 : z, hi ? f2 : z);
+        }
       }
       y = f;
     }
@@ -284,6 +314,11 @@ struct Team {
       yr = xfms(xfms(er.f, er.a, u0), er.c, up);
     }
     return y;
+  }
+  // lane 0's slot of the in-wave PCR buffer
+  __device__ Slot<R> *pcr_buf() {
+    cx<R> *u = reinterpret_cast<cx<R> *>(lds + 2 * (T + 2 * kPad)) + cr_values(T);
+    return reinterpret_cast<Slot<R> *>(u) + kPad;
   }
   // lane 0's slot of the next buffer
   __device__ Slot<R> *next_buf() {
@@ -498,11 +533,23 @@ __device__ void fill_media(const MedCache<M> &mc, const cpr_ray &P, const cpr_en
   }
 }
 
+// Diagonal Galerkin entry of the rows past nz in complex64: |q4 + d2|^2
+// overflows FP32, so the reciprocal pivot of such a row is exactly zero and
+// the row's elimination values vanish, as the masked pivot of complex128
+// makes them (a mask per row would cost a lane-mask register per row).
+constexpr float kPadPivot = 1e30f;
+
 // Coefficients of lane row k (global row g0+k) from the cached media, water
 // for rows <= iz (matrc.py:63-107) and the Galerkin stencil of
-// fused_kernel.py:43-58.
+// fused_kernel.py:43-58.  complex64 rows past nz get (0, kPadPivot, 0).
 template <class R, int M>
-__device__ CoefRow<R> coef_from_cache(const MedCache<M> &mc, int k, int g0, int iz, double cfact) {
+__device__ CoefRow<R> coef_from_cache(const MedCache<M> &mc, int k, int g0, int iz, int nz, double cfact) {
+  if (std::is_same<R, float>::value && g0 + k > nz) {
+    CoefRow<R> pad;
+    pad.d1 = pad.d3 = cxm<R>(0, 0);
+    pad.d2 = cxm<R>(static_cast<R>(kPadPivot), 0);
+    return pad;
+  }
   const double dfact = 1.0 / 12.0;
   const int o0 = (g0 + k - 1 <= iz) ? 0 : 5;
   const int o1 = (g0 + k <= iz) ? 0 : 5;
@@ -582,17 +629,17 @@ __device__ void load_coefs(Coefs<R, M, false> &C, const CoefHalf<R> *re, const C
 
 // Re-form the coefficients of lane rows klo..khi (clamped to the lane).
 template <class R, int M, bool DS>
-__device__ void rebuild_rows(Coefs<R, M, DS> &C, const MedCache<M> &mc, int g0, int iz, double cfact, int klo,
-                             int khi) {
+__device__ void rebuild_rows(Coefs<R, M, DS> &C, const MedCache<M> &mc, int g0, int iz, int nz, double cfact,
+                             int klo, int khi) {
   klo = klo < 0 ? 0 : klo;
   khi = khi > M - 1 ? M - 1 : khi;
   if constexpr (DS) {
 #pragma unroll 1
-    for (int k = klo; k <= khi; ++k) C.put(k, coef_from_cache<R, M>(mc, k, g0, iz, cfact));
+    for (int k = klo; k <= khi; ++k) C.put(k, coef_from_cache<R, M>(mc, k, g0, iz, nz, cfact));
   } else {
     CoefRow<R> tmp[M];
 #pragma unroll 1
-    for (int k = klo; k <= khi; ++k) tmp[k] = coef_from_cache<R, M>(mc, k, g0, iz, cfact);
+    for (int k = klo; k <= khi; ++k) tmp[k] = coef_from_cache<R, M>(mc, k, g0, iz, nz, cfact);
 #pragma unroll
     for (int k = 0; k < M; ++k)
       if (k >= klo && k <= khi) C.r[k] = tmp[k];
@@ -615,8 +662,9 @@ __device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
 // with D.  REAL: the wavefront's coefficients are real (every row's stencil
 // lies in the water, where the wavenumber is real), so the entries q + d
 // take real operands and streamed rows read half the bytes.  Rows past nz
-// (k >= mv) get a zero reciprocal pivot, so c', w and d' vanish there and
-// stay zero through the upward sweep and the back substitution.
+// (k >= mv) get a zero reciprocal pivot (complex128: masked; complex64: from
+// their kPadPivot diagonal), so c', w and d' vanish there and stay zero
+// through the upward sweep and the back substitution.
 //
 // SC: the lane keeps, per row, e_k = d'_k + rc (u'_k + c'_k u'_{k+1}) for
 // k < M - 2, e = d' + rc u' for k = M - 2 and d' for the last row: the upward
@@ -673,7 +721,7 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coe
       rhs = cfma(q * ts, f, du);  // f = rho
     const cx<R> den = k == 0 ? r2 : cfms(r2, r1, cprev);
     R m = rcp_real<R>(den.re * den.re + den.im * den.im);
-    m = k < mv ? m : R(0);
+    if constexpr (!std::is_same<R, float>::value) m = k < mv ? m : R(0);
     const cx<R> inv = cxm<R>(den.re * m, -(den.im * m));
     const cx<R> cc = r3 * inv;
     // first row: the previous unknown is y_{t-1} itself (w = -1, c' = d' = 0)
@@ -752,11 +800,11 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
     const int lo = jz < st.iz ? jz : st.iz, hi = jz > st.iz ? jz : st.iz;
     if (full || (fl & 2)) {
       fill_media<M>(mc, P, E, a.env, st, g0);
-      rebuild_rows<R, M, DS>(C, mc, g0, st.iz, cfact, 0, M - 1);
+      rebuild_rows<R, M, DS>(C, mc, g0, st.iz, nz, cfact, 0, M - 1);
       full = false;
       st.dirty = 0;
     } else if ((fl & 1) && g0 <= hi + 1 && g0 + M - 1 >= lo) {
-      rebuild_rows<R, M, DS>(C, mc, g0, st.iz, cfact, lo - g0, hi + 1 - g0);
+      rebuild_rows<R, M, DS>(C, mc, g0, st.iz, nz, cfact, lo - g0, hi + 1 - g0);
     }
     // every row stencil of this wavefront above the seabed row: real coefficients
     const bool wreal = __builtin_amdgcn_ballot_w64(g0 + M > st.iz) == 0;
@@ -809,17 +857,38 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
       cx<R> yc, yl, yr;
       if constexpr (TW > 0) {
         // PCR over the 64-lane wavefront, unrolled so that each level's
-        // distance s is a constant
+        // distance s is a constant; complex64 exchanges through the team's
+        // LDS buffer (in-order LDS operations of one wavefront, as in
+        // solve_cr), complex128 with shuffles
+        if constexpr (sizeof(R) == 4) {
+          Slot<R> *const PB = team.lds;
 #pragma unroll
-        for (int l = 0; l < 6; ++l) {
-          const int s = 1 << l;
-          if (s >= T) break;
-          cx<R> am, cm, fm, ap, cp, fp;
-          team.pcr(s, ea, ec, ef, am, cm, fm, ap, cp, fp);
-          reduce_step(ea, ec, ef, am, cm, fm, ap, cp, fp);
+          for (int l = 0; l < 6; ++l) {
+            const int s = 1 << l;
+            PB[t] = Slot<R>{ea, ec, ef};
+            asm volatile("" ::: "memory");
+            const Slot<R> m = PB[t - s], p = PB[t + s];
+            asm volatile("" ::: "memory");
+            reduce_step(ea, ec, ef, m.a, m.c, m.f, p.a, p.c, p.f);
+          }
+          yc = ef;
+          PB[t].a = yc;
+          asm volatile("" ::: "memory");
+          yl = PB[t - 1].a;
+          yr = PB[t + 1].a;
+          asm volatile("" ::: "memory");
+        } else {
+#pragma unroll
+          for (int l = 0; l < 6; ++l) {
+            const int s = 1 << l;
+            if (s >= T) break;
+            cx<R> am, cm, fm, ap, cp, fp;
+            team.pcr(s, ea, ec, ef, am, cm, fm, ap, cp, fp);
+            reduce_step(ea, ec, ef, am, cm, fm, ap, cp, fp);
+          }
+          yc = ef;
+          team.neighbours(yc, yl, yr);
         }
-        yc = ef;
-        team.neighbours(yc, yl, yr);
       } else {
         yc = team.solve_cr(ea, ec, ef, yl, yr);
       }
@@ -1020,12 +1089,17 @@ template __global__ void k_team_terms<double>(MarchArgs, double *);
 // At least two waves per SIMD (256 VGPRs): a lone wave cannot hide the
 // latency of its shuffles and reciprocals (measured 1.6x on 8- and 12-row
 // lanes of complex64 against the 1-wave build, which took up to 314 VGPRs).
+// complex64 teams solve their reduced system through an LDS buffer of
+// 64 + 2 kPad slots each (wave_team_lds_bytes per block).
 template <class R, int M, bool DS>
 __global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, int s1) {
+  extern __shared__ char smem[];
   const int team = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 64, DS>(a, b, threadIdx.x % 64, 64, nullptr, s0, s1);
+  Slot<R> *const pb =
+      sizeof(R) == 4 ? reinterpret_cast<Slot<R> *>(smem) + (threadIdx.x / 64) * (64 + 2 * kPad) + kPad : nullptr;
+  march_team_body<R, M, 64, DS>(a, b, threadIdx.x % 64, 64, pb, s0, s1);
 }
 
 // LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64).
@@ -1044,7 +1118,8 @@ template <class R, int M, bool DS>
 int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
   if (lanes == 64) {
     const int grid = (a.team_nrays + 3) / 4;
-    hipLaunchKernelGGL((k_march_team_w<R, M, DS>), dim3(grid), dim3(256), 0, st, a, s0, s1);
+    const size_t lds = sizeof(R) == 4 ? 4 * static_cast<size_t>(64 + 2 * kPad) * sizeof(Slot<R>) : 0;
+    hipLaunchKernelGGL((k_march_team_w<R, M, DS>), dim3(grid), dim3(256), lds, st, a, s0, s1);
   } else {
     if (lanes % 64 || lanes > 512) return -1;
     const size_t lds = team_lds_bytes<R>(lanes);
