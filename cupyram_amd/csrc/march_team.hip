@@ -169,6 +169,34 @@ __device__ inline void reduce_step(cx<R> &a, cx<R> &c, cx<R> &f, cx<R> am, cx<R>
   f = fn * ib;
 }
 
+// PCR over the 64 unknowns of one wavefront (lane t holds equation t),
+// exchanging through an LDS buffer of 64 + 2 kPad slots (pb: lane 0's slot)
+// used as three arrays a, c, f of 128 complex values, each with kPad-entry
+// margins that hold zeros, the values past the ends.  A wavefront's LDS
+// operations execute in order, so a lane reads its neighbours' entries right
+// after the write of its own; the empty asm statements keep the compiler
+// from moving the accesses across one another.  Against wave shuffles, which
+// move one dword per instruction: 81.2 instead of 85.4 ms per 100-step
+// complex64 Munk launch, 10-21 % on the 4-12-row wavefront teams.
+template <class R>
+__device__ inline void pcr64_lds(Slot<R> *pb, int t, cx<R> &a, cx<R> &c, cx<R> &f) {
+  cx<R> *const A = reinterpret_cast<cx<R> *>(pb - kPad) + kPad;
+  cx<R> *const Cc = A + (64 + 2 * kPad);
+  cx<R> *const F = Cc + (64 + 2 * kPad);
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+    const int s = 1 << l;
+    A[t] = a;
+    Cc[t] = c;
+    F[t] = f;
+    asm volatile("" ::: "memory");
+    const cx<R> am = A[t - s], cm = Cc[t - s], fm = F[t - s];
+    const cx<R> ap = A[t + s], cp = Cc[t + s], fp = F[t + s];
+    asm volatile("" ::: "memory");
+    reduce_step(a, c, f, am, cm, fm, ap, cp, fp);
+  }
+}
+
 template <class R, int TW>
 struct Team {
   int t, T;
@@ -235,23 +263,7 @@ struct Team {
       // lanes past the last level (T = 448: 56 unknowns) hold y = 0
       if (t >= n) a = c = f = z;
       if constexpr (sizeof(R) == 4) {
-        // complex64: PCR through an LDS buffer of the wavefront (81.2
-        // instead of 85.4 ms per 100-step Munk launch against shuffles, which
-        // move one dword per instruction).  A wavefront's LDS operations
-        // execute in order, so a lane reads its neighbours' slots right after
-        // the write of its own; the empty asm statements keep the compiler
-        // from moving the accesses across one another.  The margins hold
-        // zeros, the values past the team's ends.
-        Slot<R> *const PB = pcr_buf();
-#pragma unroll
-        for (int l = 0; l < 6; ++l) {
-          const int s = 1 << l;
-          PB[t] = Slot<R>{a, c, f};
-          asm volatile("" ::: "memory");
-          const Slot<R> m = PB[t - s], p = PB[t + s];
-          asm volatile("" ::: "memory");
-          reduce_step(a, c, f, m.a, m.c, m.f, p.a, p.c, p.f);
-        }
+        pcr64_lds(pcr_buf(), t, a, c, f);
       } else {
         // complex128: shuffles (the 48-byte LDS slots measured slower)
 #pragma unroll
@@ -861,21 +873,13 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
         // LDS buffer (in-order LDS operations of one wavefront, as in
         // solve_cr), complex128 with shuffles
         if constexpr (sizeof(R) == 4) {
-          Slot<R> *const PB = team.lds;
-#pragma unroll
-          for (int l = 0; l < 6; ++l) {
-            const int s = 1 << l;
-            PB[t] = Slot<R>{ea, ec, ef};
-            asm volatile("" ::: "memory");
-            const Slot<R> m = PB[t - s], p = PB[t + s];
-            asm volatile("" ::: "memory");
-            reduce_step(ea, ec, ef, m.a, m.c, m.f, p.a, p.c, p.f);
-          }
+          pcr64_lds(team.lds, t, ea, ec, ef);
           yc = ef;
-          PB[t].a = yc;
+          cx<R> *const A = reinterpret_cast<cx<R> *>(team.lds - kPad) + kPad;
+          A[t] = yc;
           asm volatile("" ::: "memory");
-          yl = PB[t - 1].a;
-          yr = PB[t + 1].a;
+          yl = A[t - 1];
+          yr = A[t + 1];
           asm volatile("" ::: "memory");
         } else {
 #pragma unroll
