@@ -41,7 +41,7 @@ METRIC = "Mcell-steps/s (nz×range steps) c64 vs c128 at ≤0.5 dB TL err; HBM G
 # the same kernel (4 x SQ_ACTIVE_INST_VALU / SIMDs / busy cycles); keyed by
 # (precision, team rows, team lanes).  These are profile figures, reported
 # beside the live timing, not measured by this script.
-PROFILED = {(64, 21, 256): {"fetch": 71.53, "write": 0.22, "valu_busy": 0.862,
+PROFILED = {(64, 21, 256): {"fetch": 71.51, "write": 0.23, "valu_busy": 0.797,
                             "source": "profiles/team_march_munk10k.md"}}
 
 FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak
