@@ -172,6 +172,28 @@ int alloc_columns(cpr_plan *p, void **col, size_t rs, bool only_u = false) {
   return 0;
 }
 
+// The self-starter's FP64 columns of a plan whose march keeps its own
+// precision: stream-ordered (hipMallocAsync / hipFreeAsync on the plan's
+// stream), so the starter needs no host synchronisation and consecutive
+// plans reuse the pool's memory instead of mapping new allocations.
+int alloc_columns_async(cpr_plan *p, void **col, size_t rs) {
+  const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
+  for (int k = 0; k < 7; ++k) {
+    const size_t b = std::max<size_t>(column_cells(p, k) * sizes[k], 1);
+    HIPCHK(hipMallocAsync(&col[k], b, p->stream));
+    HIPCHK(hipMemsetAsync(col[k], 0, b, p->stream));
+  }
+  return 0;
+}
+
+void free_columns_async(cpr_plan *p, void **col) {
+  for (int k = 0; k < 7; ++k)
+    if (col[k]) {
+      (void)hipFreeAsync(col[k], p->stream);
+      col[k] = nullptr;
+    }
+}
+
 void free_columns(cpr_plan *p, void **col, size_t rs) {
   const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
   for (int k = 0; k < 7; ++k) {
@@ -444,7 +466,10 @@ int cpr_selfs(cpr_plan *p) {
   void *dcol[7] = {};
   const bool own = !(p->cfg.precision == CPR_C128 && p->variant == CPR_VARIANT_THREAD);
   if (own) {
-    if (alloc_columns(p, dcol, sizeof(double))) return -1;
+    if (alloc_columns_async(p, dcol, sizeof(double))) {
+      free_columns_async(p, dcol);
+      return -1;
+    }
     a.f1 = dcol[0];
     a.f2 = dcol[1];
     a.f3 = dcol[2];
@@ -461,19 +486,21 @@ int cpr_selfs(cpr_plan *p) {
   const int block = 64;
   const int grid = (p->nrays + block - 1) / block;
   hipLaunchKernelGGL(k_selfs, dim3(grid), dim3(block), 0, p->stream, a);
-  HIPCHK(hipGetLastError());
+  hipError_t err = hipGetLastError();
   if (own) {
-    MarchArgs m = p->args();
-    if (p->cfg.precision == CPR_C64)
-      hipLaunchKernelGGL(k_field_from_starter<float>, dim3(grid), dim3(block), 0, p->stream, m,
-                         static_cast<const cx<double> *>(dcol[4]));
-    else
-      hipLaunchKernelGGL(k_field_from_starter<double>, dim3(grid), dim3(block), 0, p->stream, m,
-                         static_cast<const cx<double> *>(dcol[4]));
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(p->stream));
-    free_columns(p, dcol, sizeof(double));
+    if (err == hipSuccess) {
+      MarchArgs m = p->args();
+      if (p->cfg.precision == CPR_C64)
+        hipLaunchKernelGGL(k_field_from_starter<float>, dim3(grid), dim3(block), 0, p->stream, m,
+                           static_cast<const cx<double> *>(dcol[4]));
+      else
+        hipLaunchKernelGGL(k_field_from_starter<double>, dim3(grid), dim3(block), 0, p->stream, m,
+                           static_cast<const cx<double> *>(dcol[4]));
+      err = hipGetLastError();
+    }
+    free_columns_async(p, dcol);
   }
+  HIPCHK(err);
   return 0;
 }
 

@@ -301,7 +301,14 @@ class CuPyRAM:
         return v.ravel()
 
     def setup(self):
-        """Resolve per-ray grids, pack the environment and the Pade sets."""
+        """Resolve per-ray grids, pack the environment and the Pade sets.
+
+        It depends on the constructor's arguments only, so the work is done
+        once: run() calls it again (as the reference's run() does) and returns
+        at once when it has been called before, for instance by a thread that
+        prepares the next batch while the GPU marches this one."""
+        if getattr(self, "_setup_done", False):
+            return
         ne, nf, nc = self._n_env, self._n_freq, self._total_batch
         dz_e = self._per_env(self._dz, ne)
         dr_e = self._per_env(self._dr, ne)
@@ -434,6 +441,7 @@ class CuPyRAM:
             self.vz = numpy.arange(1, nvz + 1) * dz_e[0] * ndz_e[0]
         self._nvr, self._nvz = nvr, nvz
         self.eta = 1 / (40 * numpy.pi * numpy.log10(numpy.exp(1)))
+        self._setup_done = True
 
     # -- run (cupyram.py:273-380) ------------------------------------------------
 
@@ -501,7 +509,12 @@ class CuPyRAM:
         cpl = numpy.zeros((nc, self._nvr), numpy.complex128)
         tlg = numpy.zeros((nc, self._nvz, self._nvr)) if self._compute_grids else None
         cpg = numpy.zeros((nc, self._nvz, self._nvr), numpy.complex128) if self._compute_grids else None
-        tlc = numpy.full((nc, self._nvz, self._nvr), numpy.nan) if self._compute_cells else None
+        tlc = None
+        if self._compute_cells:
+            # NaN marks the cells past a ray's own grid; without such rays every cell is written
+            ragged = numpy.any(self._rays["nvz"] < self._nvz) or numpy.any(self._rays["nvr"] < self._nvr)
+            dims = (nc, self._nvz, self._nvr)
+            tlc = numpy.full(dims, numpy.nan) if ragged else numpy.empty(dims)
         plans = []
         try:
             for idx, shape, variant in buckets:
@@ -528,9 +541,11 @@ class CuPyRAM:
                     tlg[idx, :g.shape[1], :nv], cpg[idx, :g.shape[1], :nv] = g, h
                 if self._compute_cells:
                     cells = plan.read_cells()
-                    for k, c in enumerate(idx):
-                        nr_c, nz_c = int(self._rays["nvr"][c]), int(self._rays["nvz"][c])
-                        tlc[c, :nz_c, :nr_c] = cells[k, :nz_c, :nr_c]
+                    # one copy per distinct (depth bins, range bins) of the bucket
+                    dims = numpy.stack([self._rays["nvz"][idx], self._rays["nvr"][idx]], axis=1)
+                    for nz_c, nr_c in numpy.unique(dims, axis=0):
+                        k = numpy.nonzero((dims[:, 0] == nz_c) & (dims[:, 1] == nr_c))[0]
+                        tlc[idx[k], :nz_c, :nr_c] = cells[k, :nz_c, :nr_c]
                 self.plan_variants.append((len(idx), plan.describe(0)))
             self.plan_variant = self.plan_variants[0][1]
         finally:

@@ -17,6 +17,7 @@ import math
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -41,19 +42,34 @@ def pe_fields(dom: Domain, q: Queries, precision="c64", chunk=10000, log=_log):
     n = len(q)
     out = torch.empty(n, dom.nr, dom.nzp, dtype=torch.float32, device="cuda")
     t_march, cells = 0.0, 0.0
-    for a in range(0, n, chunk):
+    starts = list(range(0, n, chunk))
+
+    def prepare(a):
+        # host set-up of one chunk (keyword arguments, grids, packed tables,
+        # Pade sets); ctypes releases the GIL while the march runs, so the next
+        # chunk is prepared on a second thread meanwhile
         idx = np.arange(a, min(n, a + chunk))
         m = CuPyRAM(**D.march_kwargs(dom, q.take(idx), precision=precision))
-        t0 = time.perf_counter()
-        res = m.run()
-        t_march += time.perf_counter() - t0
-        r = m._rays
-        cells += float(np.sum(r["nz"].astype(np.float64) * r["nsteps"]))
-        key = "TL Cells" if dom.target == "cells" else "TL Grid"
-        tlg = np.asarray(res[key]).reshape(len(idx), dom.nzp, dom.nr)
-        out[a:a + len(idx)] = torch.from_numpy(np.ascontiguousarray(tlg.transpose(0, 2, 1))).float().cuda()
-        if log:
-            log(f"  PE {precision} {dom.name}: {a + len(idx)}/{n} fields ({t_march:.1f} s)")
+        m.setup()
+        return idx, m
+
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        nxt = pool.submit(prepare, starts[0]) if starts else None
+        for i, a in enumerate(starts):
+            idx, m = nxt.result()
+            if i + 1 < len(starts):
+                nxt = pool.submit(prepare, starts[i + 1])
+            t0 = time.perf_counter()
+            res = m.run()
+            t_march += time.perf_counter() - t0
+            r = m._rays
+            cells += float(np.sum(r["nz"].astype(np.float64) * r["nsteps"]))
+            key = "TL Cells" if dom.target == "cells" else "TL Grid"
+            tlg = np.asarray(res[key]).reshape(len(idx), dom.nzp, dom.nr)
+            # [depth][range] per field to the network's [range][depth], converted on the GPU
+            out[a:a + len(idx)] = torch.from_numpy(tlg).cuda().transpose(1, 2).float()
+            if log:
+                log(f"  PE {precision} {dom.name}: {a + len(idx)}/{n} fields ({t_march:.1f} s)")
     return out, t_march, cells
 
 

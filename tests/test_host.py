@@ -191,6 +191,21 @@ def test_per_ray_grids():
         assert (r["nz"], r["nvr"], r["nsteps"]) == (nz, nvr, nsteps)
 
 
+def test_setup_runs_once():
+    """setup() resolves everything from the constructor's arguments, so a
+    second call (run() makes one) keeps the first call's tables: a batch
+    prepared on another thread is not set up again."""
+    m = CuPyRAM(**envs.varying_ranges())
+    m.setup()
+    rays, pade = m._rays, m._pade
+    m.setup()
+    assert m._rays is rays and m._pade is pade
+    fresh = CuPyRAM(**envs.varying_ranges())
+    fresh.setup()
+    np.testing.assert_array_equal(fresh._rays, rays)
+    np.testing.assert_array_equal(fresh._pade, pade)
+
+
 def _square(x):
     return x * x
 
