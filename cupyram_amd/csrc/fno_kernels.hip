@@ -525,28 +525,53 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
     }
   }
   // epilogue: bias, activation, bf16 output (and its transposed copy for
-  // ANA); BWD: the gradient through GELU' of pin, and the column sums
+  // ANA); BWD: the gradient through GELU' of pin, and the column sums.  The
+  // bf16 outputs are held in registers and leave through LDS tiles (below),
+  // so that every global store is a 16-byte piece of a contiguous run.
+  // bf16 outputs held as pairs of consecutive r (packed, 32 VGPRs)
+  unsigned hold[2][2][8];
+  const bool wpre = !BWD && g.pre && live, rpin = BWD && g.pin && live;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int n = wn + j * 32 + (lane & 31);
     const float bn = !BWD && g.bias ? g.bias[n] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] += bn;
+    if (wpre) {
+      // FP32 pre-activations (training), one branch for the whole tile
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          g.pre[(static_cast<int64_t>(s) * FZ + z) * FC + n] = acc[i][j][r];
+        }
+    }
+    if (rpin) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          acc[i][j][r] *= gelu_grad(g.pin[(static_cast<int64_t>(s) * FZ + z) * FC + n]);
+        }
+    }
     float cs = 0.0f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int64_t o = (static_cast<int64_t>(s) * FZ + z) * FC + n;
-        float v = acc[i][j][r] + bn;
-        if (BWD) {
-          if (g.pin && live) v *= gelu_grad(g.pin[o]);
+        float v = acc[i][j][r];
+        if (BWD)
           cs += v;
-        } else {
-          if (g.pre && live) g.pre[o] = v;
-          if (ACT) v = gelu_f(v);
-        }
+        else if (ACT)
+          v = gelu_f(v);
         const bf16 hv = static_cast<bf16>(v);
-        if (live) g.hout[o] = hv;
+        const unsigned bits = __builtin_bit_cast(unsigned short, hv);
+        hold[j][i][r >> 1] = (r & 1) ? (hold[j][i][r >> 1] | (bits << 16)) : bits;
         if (ANA) ht[sb][n][z] = live ? hv : static_cast<bf16>(0.0f);
       }
     if (BWD) {
@@ -555,10 +580,17 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
       if (live && lane < 32) g.csum[static_cast<int64_t>(s) * FC + n] = cs;
     }
   }
+  // Output tiles in LDS, written after every wave has finished reading ht
+  // (ANA) and ms (SYN): hout of the block's two columns as [sb][z][c] in the
+  // ht array (32 of its 36 KB), and the modes xh as [k][sb][re|im][c] in the
+  // ms array (16 KB).  In HBM both are contiguous runs: hout[s0 .. s0+1] is
+  // 32 KB, and xh[k][s0 .. s0+1] is 1 KB for each mode k.
+  bf16 *const htile = &ht[0][0][0];
+  bf16 *const xtile = &ms[0][0][0];
+  f32x16 a2[2];
   if (ANA) {
     __syncthreads();
     // xh[k'][s][c] = sum_z Ban[k'][z] h[s][z][c]: 32 x 64 (k' x c) per wave
-    f32x16 a2[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -577,17 +609,46 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
         }
       }
     }
-    if (live) {
+  }
+  __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int c = wn + j * 32 + (lane & 31);
+  for (int j = 0; j < 2; ++j) {
+    const int n = wn + j * 32 + (lane & 31);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int kp = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          const int k = kp % (FK2 / 2), im = kp / (FK2 / 2);
-          g.xh[(static_cast<int64_t>(k) * g.S + s) * 2 * FC + im * FC + c] = static_cast<bf16>(a2[j][r]);
-        }
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const unsigned short bits = static_cast<unsigned short>(hold[j][i][r >> 1] >> (16 * (r & 1)));
+        htile[(sb * FZ + z) * FC + n] = __builtin_bit_cast(bf16, bits);
       }
+    if (ANA) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int kp = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int k = kp % (FK2 / 2), im = kp / (FK2 / 2);
+        xtile[((k * 2 + sb) * 2 + im) * FC + n] = static_cast<bf16>(a2[j][r]);
+      }
+    }
+  }
+  __syncthreads();
+  // hout: 2 x 64 x 128 bf16 = 2048 pieces of 16 bytes, in order
+  const int64_t hbase = static_cast<int64_t>(s0) * FZ * FC;
+  const int hlive = (g.S - s0 >= 2 ? 2 : 1) * FZ * FC / 8;  // pieces of the live columns
+#pragma unroll
+  for (int it = 0; it < 8; ++it) {
+    const int q = tid + 256 * it;
+    if (q < hlive) *reinterpret_cast<bf16x8 *>(g.hout + hbase + q * 8) = *reinterpret_cast<const bf16x8 *>(htile + q * 8);
+  }
+  if (ANA) {
+    // xh: 16 modes x (2 columns x 256) bf16 = 1024 pieces, 64 per mode
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int q = tid + 256 * it;
+      const int k = q >> 6, w = q & 63;
+      if (s0 + (w >> 5) < g.S)
+        *reinterpret_cast<bf16x8 *>(g.xh + (static_cast<int64_t>(k) * g.S + s0) * 2 * FC + w * 8) =
+            *reinterpret_cast<const bf16x8 *>(xtile + q * 8);
     }
   }
 }
