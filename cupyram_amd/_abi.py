@@ -99,6 +99,7 @@ _SYMBOLS = {
                             C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_gelu_bwd": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "cpr_fno_spec_expand": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_spec_expand_t": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_spec_fold": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_colsum": ([C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_outer_gelu_bwd": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,

@@ -65,6 +65,7 @@ struct GemmArgs {
   int splits;           // split-K factor (atomic epilogue when > 1)
   int kchunk;
   int vec;              // operands allow 16-byte loads: 16-byte aligned bases, ld and batch strides % 8 == 0
+  int cvec;             // C allows 16-byte stores (bf16 C, 16-byte aligned base, ldc and sC % 8 == 0)
 };
 
 enum { EPI_F32 = 0, EPI_BF16 = 1, EPI_GELU_BF16 = 2, EPI_ATOMIC = 3 };
@@ -73,8 +74,11 @@ enum { EPI_F32 = 0, EPI_BF16 = 1, EPI_GELU_BF16 = 2, EPI_ATOMIC = 3 };
 
 template <bool TA, bool TB, int EPI>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16 As[BM][LDP];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[BN][LDP];
+  // operand tiles during the K loop; the bf16 output tile of the staged
+  // epilogue afterwards (BM x BN)
+  __shared__ __attribute__((aligned(16))) bf16 smem[BM * BN > (BM + BN) * LDP ? BM * BN : (BM + BN) * LDP];
+  bf16 (*As)[LDP] = reinterpret_cast<bf16 (*)[LDP]>(smem);
+  bf16 (*Bs)[LDP] = reinterpret_cast<bf16 (*)[LDP]>(smem + BM * LDP);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int z = blockIdx.z;
   const int batch = z / g.splits, split = z % g.splits;
@@ -182,6 +186,31 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     __syncthreads();
   }
   // epilogue: lane holds column n = lane%32 and rows 8*(r/4) + 4*(lane/32) + r%4
+  if (EPI == EPI_BF16 && g.cvec && !g.addend && m0 + BM <= g.M && n0 + BN <= g.N) {
+    // whole bf16 tile: through LDS (the K loop ended with a barrier), so
+    // that each global store is a 16-byte piece of a 256-byte row run
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int nl = wn + j * 32 + (lane & 31);
+        const float bn = g.bias ? g.bias[n0 + nl] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int ml = wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          smem[ml * BN + nl] = static_cast<bf16>(g.alpha * acc[i][j][r] + bn);
+        }
+      }
+    __syncthreads();
+    bf16 *const Cb = static_cast<bf16 *>(g.C) + batch * g.sC + static_cast<int64_t>(m0) * g.ldc + n0;
+#pragma unroll
+    for (int it = 0; it < BM * BN / 8 / 256; ++it) {
+      const int q = tid + 256 * it, ml = q / (BN / 8), n8 = (q % (BN / 8)) * 8;
+      *reinterpret_cast<bf16x8 *>(Cb + static_cast<int64_t>(ml) * g.ldc + n8) =
+          *reinterpret_cast<const bf16x8 *>(smem + ml * BN + n8);
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -307,8 +336,11 @@ __global__ void k_outer_gelu_bwd(const bf16 *dy, const float *p, const float *pr
 
 // Spectral weights: complex W[k] = Wr[k] + i Wi[k] ([K][C][C], row = input
 // channel) acts on the real row vector [Re | Im] as the block matrix
-// [[Wr, Wi], [-Wi, Wr]] ([K][2C][2C]).  expand writes the block (bf16);
-// fold maps a block gradient G to (dWr, dWi) = (G00 + G11, G01 - G10).
+// [[Wr, Wi], [-Wi, Wr]] ([K][2C][2C]).  expand writes the block (bf16), or
+// its transpose [[Wr^T, -Wi^T], [Wi^T, Wr^T]] (TR: the operand layout of a
+// product that reads 8 consecutive input channels per lane); fold maps a
+// block gradient G to (dWr, dWi) = (G00 + G11, G01 - G10).
+template <bool TR>
 __global__ void k_spec_expand(const float *wr, const float *wi, bf16 *blk, int K, int C) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t n = static_cast<int64_t>(K) * C * C;
@@ -316,10 +348,17 @@ __global__ void k_spec_expand(const float *wr, const float *wi, bf16 *blk, int K
   const int k = static_cast<int>(i / (C * C)), r = static_cast<int>((i / C) % C), c = static_cast<int>(i % C);
   bf16 *b = blk + static_cast<int64_t>(k) * 4 * C * C;
   const int L = 2 * C;
-  b[r * L + c] = static_cast<bf16>(wr[i]);
-  b[r * L + C + c] = static_cast<bf16>(wi[i]);
-  b[(C + r) * L + c] = static_cast<bf16>(-wi[i]);
-  b[(C + r) * L + C + c] = static_cast<bf16>(wr[i]);
+  if (TR) {
+    b[c * L + r] = static_cast<bf16>(wr[i]);
+    b[(C + c) * L + r] = static_cast<bf16>(wi[i]);
+    b[c * L + C + r] = static_cast<bf16>(-wi[i]);
+    b[(C + c) * L + C + r] = static_cast<bf16>(wr[i]);
+  } else {
+    b[r * L + c] = static_cast<bf16>(wr[i]);
+    b[r * L + C + c] = static_cast<bf16>(wi[i]);
+    b[(C + r) * L + c] = static_cast<bf16>(-wi[i]);
+    b[(C + r) * L + C + c] = static_cast<bf16>(wr[i]);
+  }
 }
 
 __global__ void k_spec_fold(const float *g, float *gr, float *gi, int K, int C) {
@@ -572,7 +611,7 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
         const bf16 hv = static_cast<bf16>(v);
         const unsigned bits = __builtin_bit_cast(unsigned short, hv);
         hold[j][i][r >> 1] = (r & 1) ? (hold[j][i][r >> 1] | (bits << 16)) : bits;
-        if (ANA) ht[sb][n][z] = live ? hv : static_cast<bf16>(0.0f);
+        if (ANA) ht[sb][n][z] = hv;  // a column past S is analysed but not stored
       }
     if (BWD) {
       // lanes l and l ^ 32 hold the two z halves of column n
@@ -770,6 +809,7 @@ extern "C" int cpr_fno_gemm(int ta, int tb, int epi, int batch, int M, int N, in
   g.kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
   const auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
   g.vec = al16(A) && al16(B) && lda % 8 == 0 && ldb % 8 == 0 && sA % 8 == 0 && sB % 8 == 0 && g.kchunk % 8 == 0;
+  g.cvec = epi == EPI_BF16 && al16(C) && ldc % 8 == 0 && sC % 8 == 0;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (!ta && !tb) return launch_gemm_epi<false, false>(g, epi, batch, st);
   if (!ta && tb) return launch_gemm_epi<false, true>(g, epi, batch, st);
@@ -808,9 +848,17 @@ extern "C" int cpr_fno_gelu_bwd(const float *dy, const float *pre, void *dx, int
 extern "C" int cpr_fno_spec_expand(const float *wr, const float *wi, void *blk, int K, int C, void *stream) {
   if (K <= 0 || C <= 0) return cpr_fail("cpr_fno_spec_expand: invalid arguments");
   const int64_t n = static_cast<int64_t>(K) * C * C;
-  hipLaunchKernelGGL(k_spec_expand, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), wr, wi,
-                     static_cast<bf16 *>(blk), K, C);
+  hipLaunchKernelGGL(k_spec_expand<false>, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     wr, wi, static_cast<bf16 *>(blk), K, C);
   return launched("cpr_fno_spec_expand");
+}
+
+extern "C" int cpr_fno_spec_expand_t(const float *wr, const float *wi, void *blk_t, int K, int C, void *stream) {
+  if (K <= 0 || C <= 0) return cpr_fail("cpr_fno_spec_expand_t: invalid arguments");
+  const int64_t n = static_cast<int64_t>(K) * C * C;
+  hipLaunchKernelGGL(k_spec_expand<true>, dim3(blocks_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     wr, wi, static_cast<bf16 *>(blk_t), K, C);
+  return launched("cpr_fno_spec_expand_t");
 }
 
 extern "C" int cpr_fno_spec_fold(const float *g, float *gr, float *gi, int K, int C, void *stream) {

@@ -115,6 +115,10 @@ class FNO:
         Km, Cw = self.cfg.modes, self.cfg.width
         self.blk = {l: torch.empty(Km, 2 * Cw, 2 * Cw, dtype=torch.bfloat16, device=self.device)
                     for l in range(1, self.cfg.layers + 1)}
+        # the transposed blocks: operand of the forward mode mixing (a product
+        # with transposed B stages its tiles with 16-byte loads)
+        self.blk_t = {l: torch.empty(Km, 2 * Cw, 2 * Cw, dtype=torch.bfloat16, device=self.device)
+                      for l in range(1, self.cfg.layers + 1)}
         self.gblk = {l: torch.zeros(Km, 2 * Cw, 2 * Cw, dtype=torch.float32, device=self.device)
                      for l in range(1, self.cfg.layers + 1)}
 
@@ -131,8 +135,15 @@ class FNO:
             k.cast_bf16(self.params[n], t)
         self._fused = None  # output bias of the fused projection, read again on next use
         self._wt = None     # transposed weights of the fused kernels, rebuilt on next use
+        self._expand()
+
+    def _expand(self):
+        """Real blocks of the spectral weights (and their transposes) from the
+        FP32 masters."""
         for l in range(1, self.cfg.layers + 1):
-            k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes, self.cfg.width)
+            wr, wi = self.params[f"Wr{l}"], self.params[f"Wi{l}"]
+            k.spec_expand(wr, wi, self.blk[l], self.cfg.modes, self.cfg.width)
+            k.spec_expand(wr, wi, self.blk_t[l], self.cfg.modes, self.cfg.width, transposed=True)
 
     def load_state_dict(self, d):
         for n in self.names:
@@ -193,7 +204,7 @@ class FNO:
         for l in range(1, L + 1):
             xh = self._buf(f"xh{l}" if train else "xh", (Km, S, 2 * Cw), bf)
             k.spec_analysis(h[-1], xh, self.ac, self.as_, S, Z, Cw, Km)
-            k.gemm(xh, self.blk[l], mh, S, 2 * Cw, 2 * Cw, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+            k.gemm(xh, self.blk_t[l], mh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
                    sc=S * 2 * Cw, epi=k.EPI_BF16)
             k.spec_synth(mh, y, self.sc, self.ss, S, Z, Cw, Km)
             hn = self._buf(f"h{l}" if train else f"h{l % 2}", (R, Cw), bf)
@@ -225,7 +236,7 @@ class FNO:
         k.fno_layer(x, wt["W0"], self.params["b0"], h[0], S, ban=self.ban, xh=xh, act=False)
         mh = self._buf("mh", (Km, S, 2 * Cw), bf)
         for l in range(1, L + 1):
-            k.gemm(xh, self.blk[l], mh, S, 2 * Cw, 2 * Cw, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+            k.gemm(xh, self.blk_t[l], mh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
                    sc=S * 2 * Cw, epi=k.EPI_BF16)
             last = l == L
             xh = None if last else self._buf(f"xh{l + 1}", (Km, S, 2 * Cw), bf)
@@ -266,7 +277,7 @@ class FNO:
         mh = self._buf("fmh", (Km, S, 2 * Cw), bf)
         k.fno_layer(x, wt["W0"], self.params["b0"], h[0], S, ban=self.ban, xh=xh, act=False)
         for l in range(1, L + 1):
-            k.gemm(xh, self.blk[l], mh, S, 2 * Cw, 2 * Cw, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+            k.gemm(xh, self.blk_t[l], mh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
                    sc=S * 2 * Cw, epi=k.EPI_BF16)
             last = l == L
             k.fno_layer(h[(l - 1) % 2], wt[f"Wp{l}"], self.params[f"bp{l}"], h[l % 2], S, mh=mh,
@@ -388,9 +399,7 @@ class FNO:
         for n in self.names:
             k.adam(self.params[n], self.grads[n], self.m1[n], self.m2[n], self.bf.get(n), lr, self.step_count,
                    b1=b1, b2=b2, eps=eps, wd=0.0 if n.startswith(("b", "c")) else wd)
-        for l in range(1, self.cfg.layers + 1):
-            k.spec_expand(self.params[f"Wr{l}"], self.params[f"Wi{l}"], self.blk[l], self.cfg.modes,
-                          self.cfg.width)
+        self._expand()
 
 
 def infer(net, x):

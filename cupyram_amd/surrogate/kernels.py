@@ -101,11 +101,14 @@ def gelu_bwd(dy, pre, dx):
     _abi.check(_abi.lib().cpr_fno_gelu_bwd(_p(dy), _p(pre), _p(dx), n, _stream()))
 
 
-def spec_expand(wr, wi, blk, K, Cw):
+def spec_expand(wr, wi, blk, K, Cw, transposed=False):
+    """Real block [[Wr, Wi], [-Wi, Wr]] of the complex mode weights, or its
+    transpose."""
     _need(wr, torch.float32, K * Cw * Cw, "spec_expand wr")
     _need(wi, torch.float32, K * Cw * Cw, "spec_expand wi")
     _need(blk, torch.bfloat16, K * 4 * Cw * Cw, "spec_expand blk")
-    _abi.check(_abi.lib().cpr_fno_spec_expand(_p(wr), _p(wi), _p(blk), K, Cw, _stream()))
+    fn = _abi.lib().cpr_fno_spec_expand_t if transposed else _abi.lib().cpr_fno_spec_expand
+    _abi.check(fn(_p(wr), _p(wi), _p(blk), K, Cw, _stream()))
 
 
 def spec_fold(g, gr, gi, K, Cw):

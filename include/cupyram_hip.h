@@ -196,6 +196,9 @@ int cpr_fno_gelu_bwd(const float *dy, const float *pre, void *dx, int64_t n, voi
  * [K][2C][2C]: expand writes the bf16 block, fold maps a block gradient to
  * the gradients of Wr and Wi. */
 int cpr_fno_spec_expand(const float *wr, const float *wi, void *blk, int K, int C, void *stream);
+/* The transposed block [K][2C][2C] (blk_t[k][n][m] = blk[k][m][n]): the
+ * operand of the forward mode mixing as a product with transposed B. */
+int cpr_fno_spec_expand_t(const float *wr, const float *wi, void *blk_t, int K, int C, void *stream);
 int cpr_fno_spec_fold(const float *g, float *gr, float *gi, int K, int C, void *stream);
 /* out[n] += sum_m x[m][n], x [M][N] bf16 (or f32 if x_is_f32). */
 int cpr_fno_colsum(const void *x, int x_is_f32, float *out, int64_t M, int N, void *stream);

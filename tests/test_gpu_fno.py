@@ -115,7 +115,9 @@ def test_spectral_block_expand_and_fold(require_gpu):
     g = torch.Generator().manual_seed(6)
     wr, wi = _rand(g, K, C, C).float(), _rand(g, K, C, C).float()
     blk = torch.empty(K, 2 * C, 2 * C, dtype=torch.bfloat16, device="cuda")
+    blk_t = torch.empty_like(blk)
     k.spec_expand(wr.cuda(), wi.cuda(), blk, K, C)
+    k.spec_expand(wr.cuda(), wi.cuda(), blk_t, K, C, transposed=True)
     G = _rand(g, K, 2 * C, 2 * C).float()
     gr = torch.empty(K, C, C, device="cuda")
     gi = torch.empty(K, C, C, device="cuda")
@@ -123,6 +125,7 @@ def test_spectral_block_expand_and_fold(require_gpu):
     torch.cuda.synchronize()
     want = torch.cat([torch.cat([wr, wi], 2), torch.cat([-wi, wr], 2)], 1).to(torch.bfloat16)
     assert torch.equal(blk.cpu(), want)
+    assert torch.equal(blk_t.cpu(), want.transpose(1, 2))
     assert torch.equal(gr.cpu(), G[:, :C, :C] + G[:, C:, C:])
     assert torch.equal(gi.cpu(), G[:, :C, C:] - G[:, C:, :C])
 
