@@ -89,6 +89,7 @@ struct cpr_plan {
   double *mbuf_d = nullptr;   // team media cache
   char *qtab_d = nullptr;     // team per-term constants
   double *stage_d = nullptr;  // staging buffer of the ray-major output reads
+  void *starter_cols[7] = {};  // the self-starter's FP64 columns, freed at the next synchronisation
   int64_t stage_n = 0;        // its size in doubles
   int variant = CPR_VARIANT_THREAD;
   int team_rows = 0, team_lanes = 0;
@@ -172,28 +173,6 @@ int alloc_columns(cpr_plan *p, void **col, size_t rs, bool only_u = false) {
   return 0;
 }
 
-// The self-starter's FP64 columns of a plan whose march keeps its own
-// precision: stream-ordered (hipMallocAsync / hipFreeAsync on the plan's
-// stream), so the starter needs no host synchronisation and consecutive
-// plans reuse the pool's memory instead of mapping new allocations.
-int alloc_columns_async(cpr_plan *p, void **col, size_t rs) {
-  const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
-  for (int k = 0; k < 7; ++k) {
-    const size_t b = std::max<size_t>(column_cells(p, k) * sizes[k], 1);
-    HIPCHK(hipMallocAsync(&col[k], b, p->stream));
-    HIPCHK(hipMemsetAsync(col[k], 0, b, p->stream));
-  }
-  return 0;
-}
-
-void free_columns_async(cpr_plan *p, void **col) {
-  for (int k = 0; k < 7; ++k)
-    if (col[k]) {
-      (void)hipFreeAsync(col[k], p->stream);
-      col[k] = nullptr;
-    }
-}
-
 void free_columns(cpr_plan *p, void **col, size_t rs) {
   const size_t sizes[7] = {rs, rs, rs, 2 * rs, 2 * rs, 2 * rs, 2 * rs};
   for (int k = 0; k < 7; ++k) {
@@ -202,6 +181,20 @@ void free_columns(cpr_plan *p, void **col, size_t rs) {
       p->bytes -= static_cast<int64_t>(std::max<size_t>(column_cells(p, k) * sizes[k], 1));
       col[k] = nullptr;
     }
+  }
+}
+
+// The self-starter's FP64 columns (a plan whose march keeps its own
+// precision allocates them only for the starter): freed at the plan's next
+// synchronisation (cpr_sync, cpr_march_time, the output reads, the next
+// cpr_selfs, destroy) instead of right after the starter, so the starter
+// needs no host synchronisation.  (Stream-ordered allocations from the
+// device's memory pool did the same, but the 10,000-ray complex64 march
+// that followed them ran 8 % slower: 87 instead of 81 ms per 100 steps.)
+void release_starter_columns(cpr_plan *p) {
+  if (p->starter_cols[0] || p->starter_cols[4]) {
+    (void)hipStreamSynchronize(p->stream);
+    free_columns(p, p->starter_cols, sizeof(double));
   }
 }
 
@@ -327,6 +320,7 @@ int cpr_plan_destroy(cpr_plan *p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   for (auto e : p->events) (void)hipEventDestroy(e);
   for (auto e : p->free_events) (void)hipEventDestroy(e);
+  free_columns(p, p->starter_cols, sizeof(double));
   free_columns(p, p->col, p->real_size());
   for (void *q : {static_cast<void *>(p->rays_d), static_cast<void *>(p->envs_d), static_cast<void *>(p->env_d),
                   static_cast<void *>(p->pade_d), static_cast<void *>(p->state_d), static_cast<void *>(p->tll_d),
@@ -466,8 +460,9 @@ int cpr_selfs(cpr_plan *p) {
   void *dcol[7] = {};
   const bool own = !(p->cfg.precision == CPR_C128 && p->variant == CPR_VARIANT_THREAD);
   if (own) {
-    if (alloc_columns_async(p, dcol, sizeof(double))) {
-      free_columns_async(p, dcol);
+    release_starter_columns(p);
+    if (alloc_columns(p, dcol, sizeof(double))) {
+      free_columns(p, dcol, sizeof(double));
       return -1;
     }
     a.f1 = dcol[0];
@@ -498,7 +493,7 @@ int cpr_selfs(cpr_plan *p) {
                            static_cast<const cx<double> *>(dcol[4]));
       err = hipGetLastError();
     }
-    free_columns_async(p, dcol);
+    for (int k = 0; k < 7; ++k) p->starter_cols[k] = dcol[k];
   }
   HIPCHK(err);
   return 0;
@@ -573,12 +568,14 @@ int cpr_march_graph(cpr_plan *p, int steps_per_launch) {
 int cpr_sync(cpr_plan *p) {
   if (!p) return fail("null plan");
   HIPCHK(hipStreamSynchronize(p->stream));
+  release_starter_columns(p);
   return 0;
 }
 
 int cpr_march_time(cpr_plan *p, double *ms, int *nlaunch) {
   if (!p) return fail("null plan");
   HIPCHK(hipStreamSynchronize(p->stream));
+  release_starter_columns(p);
   if (fold_events(p)) return -1;
   if (ms) *ms = p->done_ms;
   if (nlaunch) *nlaunch = p->done_launches;
@@ -646,6 +643,7 @@ int cpr_read_outputs(cpr_plan *p, double *tll, double *cpl, double *tlg, double 
   if (tlg && read_ray_major(p, p->tlg_d, tlg, nvr, nvz, 1)) return -1;
   if (cpg && read_ray_major(p, p->cpg_d, cpg, nvr, nvz, 2)) return -1;
   HIPCHK(hipStreamSynchronize(p->stream));
+  release_starter_columns(p);
   return 0;
 }
 
@@ -655,6 +653,7 @@ int cpr_read_cells(cpr_plan *p, double *tlc) {
   if (tl_post(p)) return -1;
   if (read_ray_major(p, p->tlcell_d, tlc, p->cfg.nvr_max, p->cfg.nvz_max, 1)) return -1;
   HIPCHK(hipStreamSynchronize(p->stream));
+  release_starter_columns(p);
   return 0;
 }
 

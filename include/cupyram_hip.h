@@ -102,8 +102,9 @@ int cpr_plan_create(const cpr_config *cfg, const cpr_ray *rays, const cpr_env *e
                     cpr_plan **out);
 int cpr_plan_destroy(cpr_plan *p);
 /* Self-starter (FP64) and the first output; resets the march state.
- * Asynchronous on the plan's stream (its FP64 work columns are stream-ordered
- * allocations, freed on the stream after the field is converted). */
+ * Asynchronous on the plan's stream: its FP64 work columns are freed at the
+ * plan's next synchronisation (cpr_sync, cpr_march_time, the output reads,
+ * the next cpr_selfs or cpr_plan_destroy). */
 int cpr_selfs(cpr_plan *p);
 /* March steps [step_begin, step_end) of every ray (steps past a ray's own
  * nsteps are skipped).  Asynchronous on the plan's stream. */
