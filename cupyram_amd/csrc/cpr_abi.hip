@@ -188,9 +188,7 @@ void free_columns(cpr_plan *p, void **col, size_t rs) {
 // precision allocates them only for the starter): freed at the plan's next
 // synchronisation (cpr_sync, cpr_march_time, the output reads, the next
 // cpr_selfs, destroy) instead of right after the starter, so the starter
-// needs no host synchronisation.  (Stream-ordered allocations from the
-// device's memory pool did the same, but the 10,000-ray complex64 march
-// that followed them ran 8 % slower: 87 instead of 81 ms per 100 steps.)
+// needs no host synchronisation.
 void release_starter_columns(cpr_plan *p) {
   if (p->starter_cols[0] || p->starter_cols[4]) {
     (void)hipStreamSynchronize(p->stream);
