@@ -474,14 +474,14 @@ def main(argv=None):
     # after 50-200 ms idle a 20-step launch measured 21.1 ms instead of 18.9.
     t64 = MarchTimer(CuPyRAM(**kw, precision="c64", variant=args.variant))
     t128 = None if args.no_c128 else MarchTimer(CuPyRAM(**kw, precision="c128", variant=args.variant))
+    # The plans run on streams of their own and the self-starter is
+    # asynchronous: the complex128 warm-up completes before the complex64 one
+    # starts, so that the complex64 timing follows its own warm-up steps and
+    # does not share the GPU with other work.
     if t128:
         t128.warm(args.warmup)
-    t64.warm(args.warmup)
-    # the plans run on streams of their own and the self-starter is
-    # asynchronous: wait for both before the complex64 timing starts, so
-    # that it does not share the GPU with the complex128 warm-up
-    if t128:
         t128.plan.sync()
+    t64.warm(args.warmup)
     t64.plan.sync()
     r64 = t64.timed(args.steps, args.launch_steps)
     out_c128 = None
