@@ -370,8 +370,25 @@ struct Team {
       from_next = B[t + 1].c;
     }
   }
+  // LX: a complex64 wavefront team exchanges through its PCR buffer (in-order
+  // LDS operations of one wavefront; the entries past lane 63 are zero):
+  // 1.5-3.5 % faster than shuffles for the 4-12-row lanes, 1.5 % slower for
+  // 24 rows
+  template <bool LX>
   __device__ void next3(cx<R> a, cx<R> b, cx<R> c, cx<R> &na, cx<R> &nb, cx<R> &nc) {
-    if constexpr (TW > 0) {
+    if constexpr (TW > 0 && sizeof(R) == 4 && LX) {
+      cx<R> *const A = reinterpret_cast<cx<R> *>(lds - kPad) + kPad;
+      cx<R> *const Bb = A + (64 + 2 * kPad);
+      cx<R> *const Cc = Bb + (64 + 2 * kPad);
+      A[t] = a;
+      Bb[t] = b;
+      Cc[t] = c;
+      asm volatile("" ::: "memory");
+      na = A[t + 1];
+      nb = Bb[t + 1];
+      nc = Cc[t + 1];
+      asm volatile("" ::: "memory");
+    } else if constexpr (TW > 0) {
       const bool ok = t + 1 < T;
       const cx<R> z = cxm<R>(0, 0);
       const cx<R> ra = cxm<R>(shfl_down_r(a.re, 1, TW), shfl_down_r(a.im, 1, TW));
@@ -853,7 +870,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
       // 3. reduced equation a y_{t-1} + b y_t + c y_{t+1} = f, then PCR.  The
       // next lane's D_0 carries rc u'_0 (its first row's field, u_next here).
       cx<R> nDx, nW, nC;
-      team.next3(L.u[0], X.w(0), X.c(0), nDx, nW, nC);
+      team.template next3<!DS>(L.u[0], X.w(0), X.c(0), nDx, nW, nC);
       const cx<R> nD = SC ? cfms(nDx, rc, u_next) : nDx;
       const cx<R> ce = X.c(M - 1);
       cx<R> ea = t > 0 ? X.w(M - 1) : cxm<R>(0, 0);
