@@ -45,6 +45,19 @@ def munk_speed(z, c0=1500.0, eps=0.00737, z_axis=1300.0, scale=1300.0):
     return c0 * (1.0 + eps * (eta + np.exp(-eta) - 1.0))
 
 
+def munk_phase2(rmax=50000.0):
+    """Munk profile test of tests/test_munk.py:47-59,120-193: 25 Hz, axis
+    1000 m, 100 sound-speed knots over 5000 m, zs = zr = 1000 m, a 100 m
+    1700 m/s bottom layer, dz = 6 m, dr = 50 m, np = 8, ns = 1."""
+    z = np.linspace(0.0, 5000.0, 100)
+    return dict(freq=25.0, zs=1000.0, zr=1000.0,
+                z_ss=z, rp_ss=np.array([0.0]), cw=munk_speed(z, z_axis=1000.0).reshape(-1, 1),
+                z_sb=np.array([5000.0, 5100.0]), rp_sb=np.array([0.0]), cb=np.array([[1700.0], [1700.0]]),
+                rhob=np.array([[1.8], [1.8]]), attn=np.array([[0.5], [0.5]]),
+                rbzb=np.array([[0.0, 5000.0], [rmax, 5000.0]]),
+                dz=0.1 * 1500 / 25.0, dr=50.0, rmax=rmax, np=8, ns=1)
+
+
 def munk_batching(rmax=50000.0):
     """Munk fixture of tests/test_batching.py:184-214 (axis 1300 m, 100 Hz)."""
     z = np.linspace(0, 5000, 51)

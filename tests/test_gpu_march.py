@@ -93,6 +93,27 @@ def test_low_frequency_factor_scales(require_gpu, variant):
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_munk_phase2_no_range_drift(require_gpu, variant, precision):
+    """The reference's Munk test (tests/test_munk.py:295-330): complex128 TL
+    line within its 0.1 dB of the (Py)RAM reference everywhere (complex64:
+    0.5 dB RMS), and no range drift: the mean |dTL| over the last quarter of
+    the ranges exceeds that of the first quarter by less than 0.5 dB."""
+    m, res = _run(envs.munk_phase2(), precision=precision, variant=variant, compute_grids=False)
+    o = O.run(envs.oracle_for(m), grids=False)
+    tl = np.asarray(res["TL Line"])
+    n = min(tl.size, o["TL Line"].size)
+    d = np.abs(tl[:n] - o["TL Line"][:n])
+    drift = d[3 * n // 4:].mean() - d[:n // 4].mean()
+    print(f"{precision} {variant}: max {d.max():.3e} dB, rms {np.sqrt(np.mean(d * d)):.3e} dB, drift {drift:+.3e} dB")
+    if precision == "c128":
+        assert d.max() <= 0.1
+    else:
+        assert np.sqrt(np.mean(d * d)) <= 0.5
+    assert abs(drift) < 0.5
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
 def test_munk_short(require_gpu, variant):
     kw = envs.munk_batching(rmax=5000.0)
     _compare_with_oracle(kw, "c128", variant, 0.1, grids=False)
