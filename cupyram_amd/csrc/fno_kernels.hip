@@ -20,6 +20,8 @@
 
 typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
@@ -598,20 +600,24 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
         }
     }
     float cs = 0.0f;
+    // rows r, r+1 of a lane are depths z, z+1: converted as one pair and, for
+    // the transposed tile, written as one 4-byte word at a constant offset
+    unsigned *const hb = reinterpret_cast<unsigned *>(&ht[sb][n][4 * (lane >> 5)]);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        float v = acc[i][j][r];
-        if (BWD)
-          cs += v;
-        else if (ACT)
-          v = gelu_f(v);
-        const bf16 hv = static_cast<bf16>(v);
-        const unsigned bits = __builtin_bit_cast(unsigned short, hv);
-        hold[j][i][r >> 1] = (r & 1) ? (hold[j][i][r >> 1] | (bits << 16)) : bits;
-        if (ANA) ht[sb][n][z] = hv;  // a column past S is analysed but not stored
+      for (int rp = 0; rp < 8; ++rp) {
+        f32x2 v = {acc[i][j][2 * rp], acc[i][j][2 * rp + 1]};
+        if (BWD) {
+          cs += v[0] + v[1];
+        } else if (ACT) {
+          v[0] = gelu_f(v[0]);
+          v[1] = gelu_f(v[1]);
+        }
+        const unsigned pair = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+        hold[j][i][rp] = pair;
+        // a column past S is analysed but not stored
+        if (ANA) hb[(i * 32 + 8 * (rp >> 1) + 2 * (rp & 1)) / 2] = pair;
       }
     if (BWD) {
       // lanes l and l ^ 32 hold the two z halves of column n
@@ -653,20 +659,26 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const int n = wn + j * 32 + (lane & 31);
+    // per-lane bases, so that every write below takes a constant offset
+    unsigned short *const tb = reinterpret_cast<unsigned short *>(htile) + (sb * FZ + 4 * (lane >> 5)) * FC + n;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int z = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const unsigned short bits = static_cast<unsigned short>(hold[j][i][r >> 1] >> (16 * (r & 1)));
-        htile[(sb * FZ + z) * FC + n] = __builtin_bit_cast(bf16, bits);
-      }
+      for (int r = 0; r < 16; ++r)
+        tb[(i * 32 + 8 * (r >> 2) + (r & 3)) * FC] = static_cast<unsigned short>(hold[j][i][r >> 1] >> (16 * (r & 1)));
     if (ANA) {
+      // mode k' = 8 (r / 4) + 4 (lane / 32) + r % 4 is k = k' % 16 of part
+      // im = k' / 16; the lane's 4 (lane / 32) never carries into bit 4
+      bf16 *const xb = xtile + (4 * (lane >> 5) * 4 + sb * 2) * FC + n;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int kp = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int k = kp % (FK2 / 2), im = kp / (FK2 / 2);
-        xtile[((k * 2 + sb) * 2 + im) * FC + n] = static_cast<bf16>(a2[j][r]);
+      for (int rp = 0; rp < 8; ++rp) {
+        const f32x2 v = {a2[j][2 * rp], a2[j][2 * rp + 1]};
+        const bf16x2 b = __builtin_convertvector(v, bf16x2);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int c = 8 * ((2 * rp + e) >> 2) + ((2 * rp + e) & 3);
+          xb[((c & 15) * 4 + (c >> 4)) * FC] = b[e];
+        }
       }
     }
   }
