@@ -96,3 +96,50 @@ def test_encode_local_features():
     # global part is the same for every point of a query and differs between queries
     assert np.all(x[0, :, :, :N_GLOBAL] == x[0, :1, :1, :N_GLOBAL])
     assert not np.array_equal(x[0, 0, 0, :N_GLOBAL], x[1, 0, 0, :N_GLOBAL])
+
+
+def _encode_by_slices(dom, q, features=128):
+    """The input encoding written feature by feature into a float32 tensor
+    and cast once (the straightforward form of encode)."""
+    import torch
+    from cupyram_amd.surrogate.encode import N_BATHY, N_SSP, global_features
+    n = len(q)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32))  # noqa: E731
+    g = global_features(dom, q, "cpu")
+    r, z = t(dom.ranges() / dom.rmax), t(dom.depths() / dom.zplot)
+    pos = r * (N_BATHY - 1)
+    i0 = torch.clamp(pos.floor().long(), 0, N_BATHY - 2)
+    w = (pos - i0.float())[None, :]
+    bz = g[:, N_SSP:N_SSP + N_BATHY]
+    zb = bz[:, i0] * (1 - w) + bz[:, i0 + 1] * w
+    zsn = g[:, N_GLOBAL - 1]
+    out = torch.zeros(n, dom.nr, dom.nzp, features)
+    out[..., :N_GLOBAL] = g[:, None, None, :]
+    out[..., N_GLOBAL + 0] = r[None, :, None]
+    out[..., N_GLOBAL + 1] = z[None, None, :]
+    out[..., N_GLOBAL + 2] = zb[:, :, None]
+    out[..., N_GLOBAL + 3] = z[None, None, :] - zb[:, :, None]
+    out[..., N_GLOBAL + 4] = z[None, None, :] - zsn[:, None, None]
+    out[..., N_GLOBAL + 5] = (z[None, None, :] > zb[:, :, None]).float()
+    out[..., N_GLOBAL + 6] = 1.0
+    o = N_GLOBAL + N_LOCAL
+    out[..., o] = torch.log(r)[None, :, None]
+    for i, kf in enumerate(FOURIER_K):
+        out[..., o + 1 + 4 * i] = torch.sin(np.pi * kf * r)[None, :, None]
+        out[..., o + 2 + 4 * i] = torch.cos(np.pi * kf * r)[None, :, None]
+        out[..., o + 3 + 4 * i] = torch.sin(np.pi * kf * z)[None, None, :]
+        out[..., o + 4 + 4 * i] = torch.cos(np.pi * kf * z)[None, None, :]
+    return out.to(torch.bfloat16).reshape(n * dom.nr, dom.nzp, features)
+
+
+@pytest.mark.parametrize("name", ["wedge", "deep_munk"])
+def test_encode_equals_feature_by_feature_form(name):
+    """encode() assembles the bf16 input from per-feature pieces in one
+    concatenation; it must equal, bit for bit, the input written feature by
+    feature in float32 and cast once."""
+    torch = pytest.importorskip("torch")
+    dom = D.DOMAINS[name]
+    gen = D.wedge_queries if name == "wedge" else D.deep_queries
+    q = gen(4, 9)
+    assert torch.equal(encode(dom, q, device="cpu"), _encode_by_slices(dom, q))
+
