@@ -488,6 +488,12 @@ struct LayerArgs {
   float *pre;        // [S][Z][C] FP32 pre-activation acc + bias, or null (training)
   const float *pin;  // BWD: [S][Z][C] FP32 pre-activation the output gradient passes through, or null
   float *csum;       // BWD: [S][C] per-column sums of the output (bias gradient partials)
+  // PROJ: the projection fused into the last layer (h never reaches HBM)
+  const bf16 *p1t;   // [C out][C in]
+  const float *c1;   // [C]
+  const float *p2;   // [C]
+  float c2;
+  float *y;          // [S][Z]
   int S;
 };
 
@@ -510,8 +516,12 @@ __device__ inline void pointwise_tiles(const bf16 *hrows, const bf16 *wt, int wn
 
 }  // namespace
 
-template <bool SYN, bool ACT, bool ANA, bool BWD = false>
+template <bool SYN, bool ACT, bool ANA, bool BWD = false, bool PROJ = false>
 __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
+  static_assert(!(PROJ && (ANA || BWD)), "the projection follows the last forward layer only");
+  // row stride of the [sb][z][c] output tile: padded when the projection
+  // reads it back as MFMA rows (16-byte reads of 32 rows at one k)
+  constexpr int TL = PROJ ? FC + 8 : FC;
   __shared__ __attribute__((aligned(16))) bf16 ms[2][FK2][FC];    // mixed modes of the block's two columns
   __shared__ __attribute__((aligned(16))) bf16 ht[2][FC][HT_LD];  // output tile, transposed (channel-major)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -660,12 +670,12 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
   for (int j = 0; j < 2; ++j) {
     const int n = wn + j * 32 + (lane & 31);
     // per-lane bases, so that every write below takes a constant offset
-    unsigned short *const tb = reinterpret_cast<unsigned short *>(htile) + (sb * FZ + 4 * (lane >> 5)) * FC + n;
+    unsigned short *const tb = reinterpret_cast<unsigned short *>(htile) + (sb * FZ + 4 * (lane >> 5)) * TL + n;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 16; ++r)
-        tb[(i * 32 + 8 * (r >> 2) + (r & 3)) * FC] = static_cast<unsigned short>(hold[j][i][r >> 1] >> (16 * (r & 1)));
+        tb[(i * 32 + 8 * (r >> 2) + (r & 3)) * TL] = static_cast<unsigned short>(hold[j][i][r >> 1] >> (16 * (r & 1)));
     if (ANA) {
       // mode k' = 8 (r / 4) + 4 (lane / 32) + r % 4 is k = k' % 16 of part
       // im = k' / 16; the lane's 4 (lane / 32) never carries into bit 4
@@ -683,6 +693,71 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
     }
   }
   __syncthreads();
+  if constexpr (PROJ) {
+    // y[row] = GELU(h . P1 + c1) . P2 + c2 for the block's 128 rows, h read
+    // from the LDS tile; wave w takes its column's 64 rows and P1 outputs
+    // 64*(w%2) .. +63, the two halves summed through LDS
+    f32x16 pacc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) pacc[i][j][r] = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < FC / 16; ++kk) {
+      const int k = kk * 16 + 8 * (lane >> 5);
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = *reinterpret_cast<const bf16x8 *>(htile + (sb * FZ + i * 32 + (lane & 31)) * TL + k);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = ld8(g.p1t + (wn + j * 32 + (lane & 31)) * FC + k);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          pacc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], pacc[i][j], 0, 0, 0);
+    }
+    float t[2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) t[i][r] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = wn + j * 32 + (lane & 31);
+      const float cn = g.c1[n], pn = g.p2[n];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) t[i][r] += gelu_f(pacc[i][j][r] + cn) * pn;
+    }
+    // sum over the 32 channels held by lanes of the same half-wave
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        float v = t[i][r];
+#pragma unroll
+        for (int o = 16; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        t[i][r] = v;
+      }
+    float *const part = reinterpret_cast<float *>(xtile);  // [sb][channel half][z]
+    if ((lane & 31) == 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+          part[(sb * 2 + (wave & 1)) * FZ + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)] = t[i][r];
+    }
+    __syncthreads();
+    if (tid < 2 * FZ) {
+      const int b2 = tid / FZ, z = tid % FZ, ss = s0 + b2;
+      if (ss < g.S) g.y[static_cast<int64_t>(ss) * FZ + z] = part[(b2 * 2) * FZ + z] + part[(b2 * 2 + 1) * FZ + z] + g.c2;
+    }
+    return;
+  }
   // hout: 2 x 64 x 128 bf16 = 2048 pieces of 16 bytes, in order
   const int64_t hbase = static_cast<int64_t>(s0) * FZ * FC;
   const int hlive = (g.S - s0 >= 2 ? 2 : 1) * FZ * FC / 8;  // pieces of the live columns
@@ -952,7 +1027,7 @@ extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const v
                              int S, void *stream) {
   if (S <= 0 || !hin || !wt || !hout || (syn && (!mh || !bsyn)) || (ana && (!ban || !xh)))
     return cpr_fail("cpr_fno_layer: invalid arguments");
-  LayerArgs g;
+  LayerArgs g{};
   g.hin = static_cast<const bf16 *>(hin);
   g.wt = static_cast<const bf16 *>(wt);
   g.bias = bias;
@@ -978,13 +1053,35 @@ extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const v
   return launched("cpr_fno_layer");
 }
 
+extern "C" int cpr_fno_layer_project(const void *hin, const void *wt, const float *bias, const void *mh,
+                                     const void *bsyn, const void *p1t, const float *c1, const float *p2, float c2,
+                                     float *y, int S, void *stream) {
+  if (S <= 0 || !hin || !wt || !mh || !bsyn || !p1t || !c1 || !p2 || !y)
+    return cpr_fail("cpr_fno_layer_project: invalid arguments");
+  LayerArgs g{};
+  g.hin = static_cast<const bf16 *>(hin);
+  g.wt = static_cast<const bf16 *>(wt);
+  g.bias = bias;
+  g.mh = static_cast<const bf16 *>(mh);
+  g.bsyn = static_cast<const bf16 *>(bsyn);
+  g.p1t = static_cast<const bf16 *>(p1t);
+  g.c1 = c1;
+  g.p2 = p2;
+  g.c2 = c2;
+  g.y = y;
+  g.S = S;
+  hipLaunchKernelGGL((k_fno_layer<true, true, false, false, true>), dim3((S + 1) / 2), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), g);
+  return launched("cpr_fno_layer_project");
+}
+
 extern "C" int cpr_fno_layer_bwd(const void *din, const void *w, const void *dxh, const void *bsyn, const void *ban,
                                  const float *pin, void *dout, void *dmh, float *csum, int S, void *stream) {
   const bool syn = dxh != nullptr, ana = dmh != nullptr;
   if (S <= 0 || !din || !w || !dout || !csum || (syn && !bsyn) || (ana && !ban))
     return cpr_fail("cpr_fno_layer_bwd: invalid arguments");
   if (!syn && !ana) return cpr_fail("cpr_fno_layer_bwd: unsupported combination (no dxh and no dmh)");
-  LayerArgs g;
+  LayerArgs g{};
   g.hin = static_cast<const bf16 *>(din);
   g.wt = static_cast<const bf16 *>(w);
   g.bias = nullptr;

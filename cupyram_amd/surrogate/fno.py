@@ -275,15 +275,18 @@ class FNO:
         h = [self._buf("fh0", (R, Cw), bf), self._buf("fh1", (R, Cw), bf)]
         xh = self._buf("fxh", (Km, S, 2 * Cw), bf)
         mh = self._buf("fmh", (Km, S, 2 * Cw), bf)
+        pred = self._buf("fpred", (R,), torch.float32)
         k.fno_layer(x, wt["W0"], self.params["b0"], h[0], S, ban=self.ban, xh=xh, act=False)
         for l in range(1, L + 1):
             k.gemm(xh, self.blk_t[l], mh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
                    sc=S * 2 * Cw, epi=k.EPI_BF16)
-            last = l == L
-            k.fno_layer(h[(l - 1) % 2], wt[f"Wp{l}"], self.params[f"bp{l}"], h[l % 2], S, mh=mh,
-                        bsyn=self.bsyn, ban=None if last else self.ban, xh=None if last else xh, act=True)
-        pred = self._buf("fpred", (R,), torch.float32)
-        k.fno_project(h[L % 2], wt["P1"], self.params["c1"], self.params["P2"].reshape(-1), c2, pred, S)
+            if l < L:
+                k.fno_layer(h[(l - 1) % 2], wt[f"Wp{l}"], self.params[f"bp{l}"], h[l % 2], S, mh=mh,
+                            bsyn=self.bsyn, ban=self.ban, xh=xh, act=True)
+            else:
+                # the last layer's output goes straight into the projection
+                k.fno_layer_project(h[(l - 1) % 2], wt[f"Wp{l}"], self.params[f"bp{l}"], mh, self.bsyn,
+                                    wt["P1"], self.params["c1"], self.params["P2"].reshape(-1), c2, pred, S)
         return pred
 
     # -- loss and backward -----------------------------------------------

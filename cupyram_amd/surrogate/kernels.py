@@ -229,6 +229,24 @@ def fno_project(h, p1t, c1, p2, c2, y, S):
                                           _stream()))
 
 
+def fno_layer_project(hin, wt, bias, mh, bsyn, p1t, c1, p2, c2, y, S):
+    """The last fused layer and the projection in one kernel
+    (cpr_fno_layer_project): y = GELU(GELU(hin W + bias + syn(mh)) P1 + c1)
+    P2 + c2, the layer's output kept on chip."""
+    Z, Cw, K2 = FUSED_Z, FUSED_C, 2 * FUSED_K
+    _need(hin, torch.bfloat16, S * Z * Cw, "fno_layer_project hin")
+    _need(wt, torch.bfloat16, Cw * Cw, "fno_layer_project wt")
+    _need(bias, torch.float32, Cw, "fno_layer_project bias")
+    _need(mh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer_project mh")
+    _need(bsyn, torch.bfloat16, 2 * Z * K2, "fno_layer_project bsyn")
+    _need(p1t, torch.bfloat16, Cw * Cw, "fno_layer_project p1t")
+    _need(c1, torch.float32, Cw, "fno_layer_project c1")
+    _need(p2, torch.float32, Cw, "fno_layer_project p2")
+    _need(y, torch.float32, S * Z, "fno_layer_project y")
+    _abi.check(_abi.lib().cpr_fno_layer_project(_p(hin), _p(wt), _p(bias), _p(mh), _p(bsyn), _p(p1t), _p(c1),
+                                                _p(p2), C.c_float(float(c2)), _p(y), int(S), _stream()))
+
+
 def cast_bf16_t(x, y):
     """y [C][R] bf16 = x [R][C] f32 transposed."""
     R, Cc = x.shape

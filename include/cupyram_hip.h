@@ -246,6 +246,12 @@ int cpr_fno_layer_bwd(const void *din, const void *w, const void *dxh, const voi
  * ([out][in] bf16), p2 [128] f32. */
 int cpr_fno_project(const void *h, const void *p1t, const float *c1, const float *p2, float c2, float *y, int S,
                     void *stream);
+/* The last layer (cpr_fno_layer with syn = act = 1, ana = 0) followed by
+ * cpr_fno_project in one kernel: y [S*64] f32 = GELU(h . P1 + c1) . P2 + c2
+ * with h = GELU(hin . W + bias + syn(mh)) kept in LDS (inference). */
+int cpr_fno_layer_project(const void *hin, const void *wt, const float *bias, const void *mh, const void *bsyn,
+                          const void *p1t, const float *c1, const float *p2, float c2, float *y, int S,
+                          void *stream);
 int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *stream);
 /* y [Cc][R] bf16 = x [R][Cc] f32 transposed. */
 int cpr_fno_cast_bf16_t(const float *x, void *y, int R, int Cc, void *stream);

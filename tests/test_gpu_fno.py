@@ -295,6 +295,37 @@ def test_fused_inference_matches_float64_reference(require_gpu, S):
     assert e_unf < 2e-2
 
 
+@pytest.mark.parametrize("S", [6, 7])
+def test_fused_last_layer_and_projection(require_gpu, S):
+    """cpr_fno_layer_project (last layer and projection in one kernel, the
+    layer's output kept in LDS) equals the last fused layer followed by the
+    separate projection kernel up to the bf16 rounding both share."""
+    cfg, x, tgt, mask = _small_batch(14, S=S)
+    fno = FNO(cfg, seed=5)
+    for n in fno.names:
+        if n.startswith(("b", "c")):
+            fno.params[n].normal_(0.0, 0.1)
+    fno.refresh()
+    Z, Cw, Km = cfg.depth_points, cfg.width, cfg.modes
+    g = torch.Generator().manual_seed(15)
+    h = (torch.randn(S * Z, Cw, generator=g) * 0.5).to(torch.bfloat16).cuda()
+    mh = (torch.randn(Km, S, 2 * Cw, generator=g) * 0.1).to(torch.bfloat16).cuda()
+    wt = fno._transposed()
+    L = cfg.layers
+    c2 = float(fno.params["c2"].item())
+    hl = torch.empty(S * Z, Cw, dtype=torch.bfloat16, device="cuda")
+    k.fno_layer(h, wt[f"Wp{L}"], fno.params[f"bp{L}"], hl, S, mh=mh, bsyn=fno.bsyn, act=True)
+    y_sep = torch.empty(S * Z, device="cuda")
+    k.fno_project(hl, wt["P1"], fno.params["c1"], fno.params["P2"].reshape(-1), c2, y_sep, S)
+    y_one = torch.empty(S * Z, device="cuda")
+    k.fno_layer_project(h, wt[f"Wp{L}"], fno.params[f"bp{L}"], mh, fno.bsyn, wt["P1"], fno.params["c1"],
+                        fno.params["P2"].reshape(-1), c2, y_one, S)
+    torch.cuda.synchronize()
+    err = (y_one - y_sep).abs().max().item() / y_sep.abs().max().item()
+    print(f"S={S}: max |fused - separate| / max |y| = {err:.2e}")
+    assert err < 1e-5
+
+
 def test_fused_inference_follows_training_steps(require_gpu):
     """The fused path rebuilds its transposed weights after optimiser steps."""
     cfg, x, tgt, mask = _small_batch(13)
