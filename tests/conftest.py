@@ -40,5 +40,5 @@ def require_gpu():
     from cupyram_amd import _abi
     n = _abi.device_count()
     if n < 1:
-        pytest.fail("no HIP device visible; GPU tests must run on the GPU box")
+        pytest.fail("no HIP device visible; the GPU tests need an MI355X (gfx950)")
     return n
