@@ -26,7 +26,7 @@ from . import _abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning", "gfx950.json")
-ROWS = {64: (4, 8, 12, 16, 20, 21, 24, 28, 32, 40, 48, 64), 128: (2, 4, 6, 8, 10, 11, 12, 14, 16, 20, 24, 32, 40)}
+ROWS = {64: (4, 8, 12, 14, 16, 20, 21, 24, 28, 32, 40, 48, 64), 128: (2, 4, 6, 8, 10, 11, 12, 14, 16, 20, 24, 32, 40)}
 NZ_SWEEP = (200, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 5376, 6144, 7168, 8192, 10240, 12288, 14336,
             16384, 20480)
 

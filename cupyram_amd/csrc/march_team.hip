@@ -146,7 +146,7 @@ __host__ __device__ constexpr int cr_values(int T) {
 // reduced system's solution arrays, then the in-wave PCR buffer (64 slots
 // with zero margins)
 template <class R>
-constexpr size_t team_lds_bytes(int T) {
+__host__ __device__ constexpr size_t team_lds_bytes(int T) {
   return 2 * static_cast<size_t>(T + 2 * kPad) * sizeof(Slot<R>) + cr_values(T) * sizeof(cx<R>) +
          static_cast<size_t>(64 + 2 * kPad) * sizeof(Slot<R>);
 }
@@ -434,20 +434,54 @@ struct LaneCol {
 };
 
 // Elimination arrays of a lane: c' (then C) and the spike w (then W) of
-// every row.  They are register arrays at every row count: for the deepest
-// lanes (complex64 40-64 rows, complex128 16-40) the compiler spills part of
-// them to private memory, which measured 1.05-1.8x faster than streaming
-// them through a global per-team buffer.
-template <class R, int M>
+// every row.  Rows k < NR are register arrays.  In LDS teams with more rows
+// per lane than the registers hold (complex64 past 24 rows, complex128 past
+// 11), rows NR .. M-1 live in the block's LDS as a run of (c', w) pairs per
+// lane: otherwise the compiler spills them to private memory, which measured
+// up to 2x slower (complex64 32 x 512 lanes 25.2 against 52.9 G cell-steps/s
+// with the LDS rows, 28 x 512 37.1 against 55.7 G; complex128 16 x 512 13.7
+// against 18.3 G).  A lane touches only its own run; the runs are an odd
+// number of 16-byte units apart, so the lanes of a wavefront spread over every
+// bank.  Every row index is a constant after unrolling, so each access is one
+// LDS instruction at an immediate offset from the lane's base.  (Moving rows
+// of the shallower lanes to LDS measured 1-10 % slower.)
+template <class R>
+struct ElimPair {
+  cx<R> c, w;
+};
+
+template <class R, int M, int NR>
 struct Elim {
-  cx<R> cw[M], ww[M];
-  __device__ cx<R> c(int k) const { return cw[k]; }
-  __device__ cx<R> w(int k) const { return ww[k]; }
+  static_assert(NR >= 1 && NR <= M, "at least one register row");
+  cx<R> cw[NR], ww[NR];
+  ElimPair<R> *tail;  // rows NR .. M-1 of this lane (unused when NR = M)
+  __device__ cx<R> c(int k) const { return k < NR ? cw[k] : tail[k - NR].c; }
+  __device__ cx<R> w(int k) const { return k < NR ? ww[k] : tail[k - NR].w; }
   __device__ void set(int k, cx<R> cv, cx<R> wv) {
-    cw[k] = cv;
-    ww[k] = wv;
+    if (k < NR) {
+      cw[k] = cv;
+      ww[k] = wv;
+    } else {
+      tail[k - NR] = ElimPair<R>{cv, wv};
+    }
   }
 };
+
+// Register rows of the elimination arrays of an LDS team with M rows per
+// lane; the other M - NR rows (at most 15 in complex64, 5 in complex128, so
+// that a 512-lane team's LDS stays within the CU's 160 KB) go to LDS.
+template <class R, int M>
+__host__ __device__ constexpr int elim_regs() {
+  if (sizeof(R) == 4) return M <= 24 ? M : (M - 15 > 12 ? M - 15 : 12);
+  return M <= 11 ? M : M - 5;
+}
+// LDS bytes per lane of the tail rows: M - NR pairs, padded to an odd number
+// of 16-byte units
+template <class R, int M>
+__host__ __device__ constexpr int elim_tail_lane_bytes() {
+  const int units = (M - elim_regs<R, M>()) * static_cast<int>(sizeof(ElimPair<R>)) / 16;
+  return units == 0 ? 0 : (units | 1) * 16;
+}
 
 // Galerkin coefficients of one row.
 template <class R>
@@ -700,8 +734,8 @@ __device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
 // sweep D_k = e_k - c'_k D_{k+1} then yields D_k + rc u'_k (the back
 // substitution's x' without the stored u'), and the last row's u' is
 // returned in ulast for x'_{M-1} = y'_{M-1} + rc u'_{M-1}.
-template <bool REAL, bool SC, class R, int M, bool DS>
-__device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coefs<R, M, DS> &C, cx<R> q,
+template <bool REAL, bool SC, class R, int M, bool DS, class X_t>
+__device__ inline void eliminate_down(LaneCol<R, M> &L, X_t &X, const Coefs<R, M, DS> &C, cx<R> q,
                                       cx<R> q4, cx<R> f, cx<R> um, cx<R> u_next, int mv, cx<R> &ulast) {
   static_assert(M >= 2, "team lanes hold at least two rows");
   cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(0, 0), dprev = cxm<R>(0, 0);
@@ -774,7 +808,7 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, Elim<R, M> &X, const Coe
   }
 }
 
-template <class R, int M, int TW, bool DS>
+template <class R, int M, int TW, bool DS, int NR>
 __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R> *lds, int s0, int s1) {
   const cpr_ray P = make_uniform(a.rays[b]);
   const cpr_env E = make_uniform(a.envs[b]);
@@ -787,7 +821,12 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
   mv = mv < 0 ? 0 : (mv > M ? M : mv);
   LaneCol<R, M> L;
   Coefs<R, M, DS> C;
-  Elim<R, M> X;
+  Elim<R, M, NR> X;
+  if constexpr (NR < M) {
+    // the tail rows follow the team's exchange and reduced-system buffers
+    char *const base = reinterpret_cast<char *>(lds) + (team_lds_bytes<R>(T) + 15) / 16 * 16;
+    X.tail = reinterpret_cast<ElimPair<R> *>(base + t * elim_tail_lane_bytes<R, M>());
+  }
   const double cfact = 0.5 / (P.dz * P.dz);
   CoefHalf<R> *const cre = static_cast<CoefHalf<R> *>(a.dbuf) + static_cast<int64_t>(b) * M * T;
   CoefHalf<R> *const cim = static_cast<CoefHalf<R> *>(a.dbuf) + (static_cast<int64_t>(a.nrays) + b) * M * T;
@@ -1120,20 +1159,23 @@ __global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, in
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
   Slot<R> *const pb =
       sizeof(R) == 4 ? reinterpret_cast<Slot<R> *>(smem) + (threadIdx.x / 64) * (64 + 2 * kPad) + kPad : nullptr;
-  march_team_body<R, M, 64, DS>(a, b, threadIdx.x % 64, 64, pb, s0, s1);
+  march_team_body<R, M, 64, DS, M>(a, b, threadIdx.x % 64, 64, pb, s0, s1);
 }
 
-// LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64).
-template <class R, int M, bool DS>
+// LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64),
+// with the elimination rows NR .. M-1 in LDS.
+template <class R, int M, bool DS, int NR>
 __global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s1) {
   extern __shared__ char smem[];
   const int team = blockIdx.x;
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 0, DS>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<Slot<R> *>(smem), s0, s1);
+  march_team_body<R, M, 0, DS, NR>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<Slot<R> *>(smem), s0, s1);
 }
 
 namespace {
+
+constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS of one CU (gfx950)
 
 template <class R, int M, bool DS>
 int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
@@ -1143,26 +1185,44 @@ int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
     hipLaunchKernelGGL((k_march_team_w<R, M, DS>), dim3(grid), dim3(256), lds, st, a, s0, s1);
   } else {
     if (lanes % 64 || lanes > 512) return -1;
-    const size_t lds = team_lds_bytes<R>(lanes);
-    hipLaunchKernelGGL((k_march_team_l<R, M, DS>), dim3(a.team_nrays), dim3(lanes), lds, st, a, s0, s1);
+    // The LDS rows of the elimination arrays only where their LDS leaves the
+    // CU as many teams as the VGPRs allow (two waves per SIMD: eight waves per
+    // CU): a 128-lane complex128 team of 12 rows with them ran 19.7 instead of
+    // 23.1 G cell-steps/s (three blocks per CU instead of four).
+    constexpr int NR = elim_regs<R, M>();
+    const size_t base = team_lds_bytes<R>(lanes);
+    const size_t with_tail = (base + 15) / 16 * 16 + static_cast<size_t>(lanes) * elim_tail_lane_bytes<R, M>();
+    const int waves = lanes / 64;
+    const size_t blocks = waves >= 8 ? 1 : static_cast<size_t>(8 / waves);
+    if constexpr (NR < M) {
+      if (with_tail * blocks <= kMaxLdsBytes) {
+        hipLaunchKernelGGL((k_march_team_l<R, M, DS, NR>), dim3(a.team_nrays), dim3(lanes), with_tail, st, a, s0, s1);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+      }
+    }
+    if (base > kMaxLdsBytes) return -1;
+    hipLaunchKernelGGL((k_march_team_l<R, M, DS, M>), dim3(a.team_nrays), dim3(lanes), base, st, a, s0, s1);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // namespace
 
-// Rows per lane and where the lane keeps its state: the field and the
-// elimination arrays are always register arrays; complex64 keeps the Galerkin
-// coefficients in VGPRs up to 12 rows and streams them from a per-team buffer
-// from 16 rows on, complex128 up to 6 and from 8 rows on (40 x 512 lanes hold
-// the deepest columns, 20,480 rows; complex64 64 x 512 hold 32,768).
-bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 16 : rows >= 8; }
+// Rows per lane and where the lane keeps its state: the field is a register
+// array, and so are the elimination arrays except for the deep LDS teams'
+// tail rows (elim_regs); complex64 keeps the Galerkin coefficients in VGPRs
+// up to 12 rows and streams them from a per-team buffer from 14 rows on,
+// complex128 up to 6 and from 8 rows on (40 x 512 lanes hold the deepest
+// columns, 20,480 rows; complex64 64 x 512 hold 32,768).
+bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? rows >= 14 : rows >= 8; }
 
-// Rows per lane built for each precision.  Between the powers of two, 20, 21
-// and 28 (complex64) and 10 and 11 (complex128) let a column fill its lanes
-// with few padding rows (nz = 5303: 21 x 256 = 5376 rows instead of
-// 24 x 256 = 6144), or fill all eight waves of a CU (11 x 512 lanes).
-constexpr int kRows64[] = {4, 8, 12, 16, 20, 21, 24, 28, 32, 40, 48, 64};
+// Rows per lane built for each precision.  Between the powers of two, 14, 20,
+// 21 and 28 (complex64) and 10 and 11 (complex128) let a column fill its
+// lanes with few padding rows (nz = 5303: 21 x 256 = 5376 rows instead of
+// 24 x 256 = 6144), or fill all eight waves of a CU (11 x 512 lanes; complex64
+// 14 x 512 lanes for 7,168 rows: 48.7 against 43.1 G cell-steps/s for
+// 16 x 448).
+constexpr int kRows64[] = {4, 8, 12, 14, 16, 20, 21, 24, 28, 32, 40, 48, 64};
 constexpr int kRows128[] = {2, 4, 6, 8, 10, 11, 12, 14, 16, 20, 24, 32, 40};
 
 bool team_shape_supported(int precision, int rows, int lanes) {
@@ -1225,6 +1285,8 @@ int team_launch(const MarchArgs &a, int precision, int rows, int lanes, int s0, 
         return launch_rows<float, 8, false>(a, lanes, s0, s1, st);
       case 12:
         return launch_rows<float, 12, false>(a, lanes, s0, s1, st);
+      case 14:
+        return launch_rows<float, 14, true>(a, lanes, s0, s1, st);
       case 16:
         return launch_rows<float, 16, true>(a, lanes, s0, s1, st);
       case 20:
