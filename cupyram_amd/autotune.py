@@ -11,7 +11,9 @@ the library's static cost model (``cpr_team_shape``) when no table exists.
 Every candidate marches the same rays from the same start field; results are
 not compared here (the GPU tests cover every shape against the oracle).
 
-Usage: ``python -m cupyram_amd.autotune [--steps 40] [--out path]``.
+Usage: ``python -m cupyram_amd.autotune [--steps 40] [--out path] [--precision 64 128]
+[--nz depth ...] [--merge]`` (``--merge`` replaces only the swept depths'
+entries of the table at ``--out``).
 """
 from __future__ import annotations
 
@@ -136,12 +138,35 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--out", default=TABLE)
     ap.add_argument("--precision", type=int, nargs="*", default=[64, 128])
+    ap.add_argument("--nz", type=int, nargs="*", default=list(NZ_SWEEP), help="column depths to sweep")
+    ap.add_argument("--merge", action="store_true",
+                    help="replace only the swept depths' entries of the table already at --out")
     args = ap.parse_args(argv)
-    table = sweep(tuple(args.precision), steps=args.steps, log=lambda s: print(s, flush=True))
-    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    table = sweep(tuple(args.precision), nz_list=tuple(args.nz), steps=args.steps,
+                  log=lambda s: print(s, flush=True))
+    if args.merge and os.path.exists(args.out):
+        with open(args.out) as fh:
+            table = merge_tables(json.load(fh), table)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     with open(args.out, "w") as fh:
         json.dump(table, fh, indent=1)
     print("wrote", args.out)
+
+
+def merge_tables(old: dict, new: dict) -> dict:
+    """`old` with the entries of `new` replacing those of the same precision
+    and depth bound (entries sorted by depth)."""
+    out = dict(old)
+    for key, ents in new.items():
+        if not key.endswith("_mcs"):
+            continue
+        prec = key[:-4]
+        rows = {e[0]: list(e) for e in old.get(key, [])}
+        rows.update({e[0]: list(e) for e in ents})
+        merged = [rows[k] for k in sorted(rows)]
+        out[key] = merged
+        out[prec] = [[e[0], e[1]] for e in merged]
+    return out
 
 
 if __name__ == "__main__":

@@ -335,3 +335,16 @@ def test_tuning_table_and_static_shape_model(precision):
             assert rows in autotune.ROWS[precision], (nz, shape)
             assert lanes % 64 == 0 and 64 <= lanes <= 512 and rows * lanes >= nz, (nz, shape)
     assert _abi.team_shape(precision, deepest + 1) == (0, 0)
+
+
+def test_tuning_table_merge_replaces_only_swept_depths():
+    """A partial sweep (autotune --nz ... --merge) replaces the entries of its
+    precision and depths and keeps the others, sorted by depth."""
+    from cupyram_amd import autotune
+    old = {"64": [[256, 4], [512, 8], [1024, 16]], "64_mcs": [[256, 4, 1.0], [512, 8, 2.0], [1024, 16, 3.0]],
+           "128": [[512, 8]], "128_mcs": [[512, 8, 4.0]]}
+    new = {"64": [[512, 12], [2048, 16]], "64_mcs": [[512, 12, 5.0], [2048, 16, 6.0]]}
+    out = autotune.merge_tables(old, new)
+    assert out["64_mcs"] == [[256, 4, 1.0], [512, 12, 5.0], [1024, 16, 3.0], [2048, 16, 6.0]]
+    assert out["64"] == [[256, 4], [512, 12], [1024, 16], [2048, 16]]
+    assert out["128"] == old["128"] and out["128_mcs"] == old["128_mcs"]
