@@ -84,7 +84,23 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int z = blockIdx.z;
   const int batch = z / g.splits, split = z % g.splits;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  // Tile order: workgroups are handed to the eight XCDs in turn, each XCD
+  // with its own L2.  Within each run of 8 M tiles, the N tiles of one M tile
+  // get workgroup ids 8 apart, so they run on one XCD and all but the first
+  // read the A tile from its L2; M tiles past the last whole run of 8 keep the
+  // plain order.  (Bijective on the tiles of each z slice.)
+  int tx = blockIdx.x, ty = blockIdx.y;
+  if (gridDim.x > 1) {
+    const int nx = gridDim.x;
+    const int id = blockIdx.x + nx * blockIdx.y;
+    const int grp = id / (8 * nx);
+    if ((grp + 1) * 8 <= static_cast<int>(gridDim.y)) {
+      const int i = id - grp * 8 * nx;
+      ty = grp * 8 + (i & 7);
+      tx = i >> 3;
+    }
+  }
+  const int m0 = ty * BM, n0 = tx * BN;
   const bf16 *A = g.A + batch * g.sA;
   const bf16 *B = g.B + batch * g.sB;
   const int kbeg = split * g.kchunk;
