@@ -41,7 +41,7 @@ METRIC = "Mcell-steps/s (nz×range steps) c64 vs c128 at ≤0.5 dB TL err; HBM G
 # the same kernel (4 x SQ_ACTIVE_INST_VALU / SIMDs / busy cycles); keyed by
 # (precision, team rows, team lanes).  These are profile figures, reported
 # beside the live timing, not measured by this script.
-PROFILED = {(64, 21, 256): {"fetch": 71.51, "write": 0.23, "valu_busy": 0.797,
+PROFILED = {(64, 21, 256): {"fetch": 0.526, "write": 0.224, "valu_busy": 0.866,
                             "source": "profiles/team_march_munk10k.md"}}
 
 FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak
@@ -511,9 +511,9 @@ def main(argv=None):
         "config": {"workload": "munk_batch: 10,000-ray Munk 5000 m, 100 Hz, nz 5303, np 6, dr 10 m, dz 1 m",
                    "rays": args.rays, "nz": int(r64["cells"] / args.rays), "np": np_pade,
                    "variant": r64["variant"], "steps_per_launch": per_launch_steps},
-        # The team march keeps each ray's column in registers for a whole
-        # launch, so HBM carries little more than the field once per launch
-        # and the coefficient stream; its bound is VALU issue.  achieved =
+        # The team march keeps each ray's column in registers and its
+        # coefficients in LDS for a whole launch, so HBM carries little more
+        # than the field once per launch; its bound is VALU issue.  achieved =
         # 140 np FP32 flop per cell-step (the march's arithmetic) x the
         # launch's cell-steps / its hipEvent-timed duration, against the
         # published FP32 vector peak.

@@ -151,6 +151,13 @@ __host__ __device__ constexpr size_t team_lds_bytes(int T) {
          static_cast<size_t>(64 + 2 * kPad) * sizeof(Slot<R>);
 }
 
+// LDS bytes of a block of four 64-lane teams before their coefficient rows:
+// the complex64 teams' in-wave PCR buffers (complex128 teams use shuffles)
+template <class R>
+__host__ __device__ constexpr size_t wave_team_lds_bytes() {
+  return sizeof(R) == 4 ? 4 * static_cast<size_t>(64 + 2 * kPad) * sizeof(Slot<R>) : 0;
+}
+
 // One elimination step of the reduced system, shared by PCR and cyclic
 // reduction: equation (a, 1, c | f) of an unknown with the equations of its
 // neighbours at distance s, (am, 1, cm | fm) before and (ap, 1, cp | fp)
@@ -502,17 +509,19 @@ struct CoefHalf {
 // the same layout, so that a wavefront reading row k touches one contiguous
 // span of each, and a wavefront whose rows all have real coefficients (rows
 // above the seabed, see march_team_body) reads only the real half.
-template <class R, int M, bool DS>
+template <class R, int M, bool DS, int KC = 0>
 struct Coefs {
   CoefRow<R> r[M];
   __device__ CoefRow<R> get(int k) const { return r[k]; }
   __device__ CoefHalf<R> get_re(int k) const { return CoefHalf<R>{r[k].d1.re, r[k].d2.re, r[k].d3.re}; }
   __device__ const Coefs &view() const { return *this; }
 };
-template <class R, int M>
-struct Coefs<R, M, true> {
+template <class R, int M, int KC>
+struct Coefs<R, M, true, KC> {
   CoefHalf<R> *re, *im;  // team-uniform ray bases: row k of lane t at [k * T + t]
   int T, t;
+  R *lc;  // LDS (KC > 0): this lane's real halves d1 d2 d3 of rows 0 .. KC-1
+  typedef R __attribute__((address_space(3))) *lreal;
   // Load views: the ray bases copied through an empty asm statement, so the
   // compiler keeps them in scalar registers and forms each row's address
   // there (scalar base + the lane's 32-bit byte offset) instead of hoisting
@@ -523,31 +532,71 @@ struct Coefs<R, M, true> {
     gbytes re, im;
     int64_t row;   // bytes per row: T * sizeof(CoefHalf)
     unsigned off;  // this lane's byte offset: t * sizeof(CoefHalf)
+    lreal lc;
     __device__ CoefHalf<R> half(gbytes b, int k) const {
       const ghalf p = reinterpret_cast<ghalf>(b + k * row + off);
       return CoefHalf<R>{p->d1, p->d2, p->d3};
     }
+    __device__ CoefHalf<R> real(int k) const {
+      if (k < KC) return CoefHalf<R>{lc[3 * k], lc[3 * k + 1], lc[3 * k + 2]};
+      return half(re, k);
+    }
     __device__ CoefRow<R> get(int k) const {
-      const CoefHalf<R> a = half(re, k), b = half(im, k);
+      const CoefHalf<R> a = real(k), b = half(im, k);
       CoefRow<R> c;
       c.d1 = cxm<R>(a.d1, b.d1);
       c.d2 = cxm<R>(a.d2, b.d2);
       c.d3 = cxm<R>(a.d3, b.d3);
       return c;
     }
-    __device__ CoefHalf<R> get_re(int k) const { return half(re, k); }
+    __device__ CoefHalf<R> get_re(int k) const { return real(k); }
   };
   __device__ View view() const {
     gbytes r = (gbytes)(re), i = (gbytes)(im);  // generic to global address space
     asm volatile("" : "+s"(r), "+s"(i));
     return View{r, i, static_cast<int64_t>(T) * static_cast<int64_t>(sizeof(CoefHalf<R>)),
-                static_cast<unsigned>(t) * static_cast<unsigned>(sizeof(CoefHalf<R>))};
+                static_cast<unsigned>(t) * static_cast<unsigned>(sizeof(CoefHalf<R>)), (lreal)(lc)};
   }
   __device__ void put(int k, const CoefRow<R> &c) const {
     (re + k * T)[t] = CoefHalf<R>{c.d1.re, c.d2.re, c.d3.re};
     (im + k * T)[t] = CoefHalf<R>{c.d1.im, c.d2.im, c.d3.im};
+    if (k < KC) {
+      const lreal l = (lreal)(lc);
+      l[3 * k] = c.d1.re;
+      l[3 * k + 1] = c.d2.re;
+      l[3 * k + 2] = c.d3.re;
+    }
+  }
+  // copy the real halves of rows 0 .. KC-1 from the team buffer to LDS
+  __device__ void fill() const {
+    const lreal l = (lreal)(lc);
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const CoefHalf<R> h = (re + k * T)[t];
+      l[3 * k] = h.d1;
+      l[3 * k + 1] = h.d2;
+      l[3 * k + 2] = h.d3;
+    }
   }
 };
+
+// Rows per lane whose real coefficient halves a team keeps in LDS (streamed
+// modes), sized so that the blocks the VGPRs allow per CU still fit its
+// 160 KB at the lane counts the tuning table gives each row count (complex64
+// 21 x 256 lanes: 19 rows, 78 KB per block; 16 x 128: all 16 rows, 38 KB),
+// and the LDS bytes per lane they take (an odd number of values of R, so
+// that the lanes of a wavefront spread over the banks and every value is
+// aligned to its size).
+template <class R, int M>
+__host__ __device__ constexpr int coef_cache_rows() {
+  if (sizeof(R) == 4) return M <= 16 ? M : (M == 21 ? 19 : (M <= 24 ? 17 : 0));
+  return 0;
+}
+template <class R, int M>
+__host__ __device__ constexpr int coef_cache_lane_bytes() {
+  const int n = coef_cache_rows<R, M>() * 3;  // values of R
+  return n == 0 ? 0 : (n | 1) * static_cast<int>(sizeof(R));
+}
 
 // Water and bottom media of the lane's rows g0-1 .. g0+M (FP64), kept in a
 // per-plan device buffer that persists between launches, laid out
@@ -691,8 +740,8 @@ __device__ void load_coefs(Coefs<R, M, false> &C, const CoefHalf<R> *re, const C
 }
 
 // Re-form the coefficients of lane rows klo..khi (clamped to the lane).
-template <class R, int M, bool DS>
-__device__ void rebuild_rows(Coefs<R, M, DS> &C, const MedCache<M> &mc, int g0, int iz, int nz, double cfact,
+template <class R, int M, bool DS, int KC>
+__device__ void rebuild_rows(Coefs<R, M, DS, KC> &C, const MedCache<M> &mc, int g0, int iz, int nz, double cfact,
                              int klo, int khi) {
   klo = klo < 0 ? 0 : klo;
   khi = khi > M - 1 ? M - 1 : khi;
@@ -734,8 +783,8 @@ __device__ inline cx<R> pick(const cx<R> (&v)[M], int k) {
 // sweep D_k = e_k - c'_k D_{k+1} then yields D_k + rc u'_k (the back
 // substitution's x' without the stored u'), and the last row's u' is
 // returned in ulast for x'_{M-1} = y'_{M-1} + rc u'_{M-1}.
-template <bool REAL, bool SC, class R, int M, bool DS, class X_t>
-__device__ inline void eliminate_down(LaneCol<R, M> &L, X_t &X, const Coefs<R, M, DS> &C, cx<R> q,
+template <bool REAL, bool SC, class R, int M, class C_t, class X_t>
+__device__ inline void eliminate_down(LaneCol<R, M> &L, X_t &X, const C_t &C, cx<R> q,
                                       cx<R> q4, cx<R> f, cx<R> um, cx<R> u_next, int mv, cx<R> &ulast) {
   static_assert(M >= 2, "team lanes hold at least two rows");
   cx<R> cprev = cxm<R>(0, 0), wprev = cxm<R>(0, 0), dprev = cxm<R>(0, 0);
@@ -808,8 +857,8 @@ __device__ inline void eliminate_down(LaneCol<R, M> &L, X_t &X, const Coefs<R, M
   }
 }
 
-template <class R, int M, int TW, bool DS, int NR>
-__device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R> *lds, int s0, int s1) {
+template <class R, int M, int TW, bool DS, int NR, int KC>
+__device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R> *lds, R *lc, int s0, int s1) {
   const cpr_ray P = make_uniform(a.rays[b]);
   const cpr_env E = make_uniform(a.envs[b]);
   RayState st = make_uniform(a.state[b]);
@@ -820,7 +869,7 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
   int mv = nz - g0 + 1;  // valid rows of this lane
   mv = mv < 0 ? 0 : (mv > M ? M : mv);
   LaneCol<R, M> L;
-  Coefs<R, M, DS> C;
+  Coefs<R, M, DS, KC> C;
   Elim<R, M, NR> X;
   if constexpr (NR < M) {
     // the tail rows follow the team's exchange and reduced-system buffers
@@ -835,6 +884,8 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
     C.im = cim;
     C.T = T;
     C.t = t;
+    C.lc = lc;
+    if constexpr (KC > 0) C.fill();
   }
   const MedCache<M> mc{static_cast<double *>(a.mbuf) + static_cast<int64_t>(b) * (M + 2) * 10 * T + t, T};
   // the field in the team layout [ray][k][lane] (rows 1 + t M + k): a team's
@@ -1151,7 +1202,7 @@ template __global__ void k_team_terms<double>(MarchArgs, double *);
 // lanes of complex64 against the 1-wave build, which took up to 314 VGPRs).
 // complex64 teams solve their reduced system through an LDS buffer of
 // 64 + 2 kPad slots each (wave_team_lds_bytes per block).
-template <class R, int M, bool DS>
+template <class R, int M, bool DS, int KC>
 __global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, int s1) {
   extern __shared__ char smem[];
   const int team = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
@@ -1159,18 +1210,24 @@ __global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, in
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
   Slot<R> *const pb =
       sizeof(R) == 4 ? reinterpret_cast<Slot<R> *>(smem) + (threadIdx.x / 64) * (64 + 2 * kPad) + kPad : nullptr;
-  march_team_body<R, M, 64, DS, M>(a, b, threadIdx.x % 64, 64, pb, s0, s1);
+  R *const lc = KC > 0 ? reinterpret_cast<R *>(smem + wave_team_lds_bytes<R>() + threadIdx.x * coef_cache_lane_bytes<R, M>())
+                       : nullptr;
+  march_team_body<R, M, 64, DS, M, KC>(a, b, threadIdx.x % 64, 64, pb, lc, s0, s1);
 }
 
 // LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64),
 // with the elimination rows NR .. M-1 in LDS.
-template <class R, int M, bool DS, int NR>
+template <class R, int M, bool DS, int NR, int KC>
 __global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s1) {
   extern __shared__ char smem[];
   const int team = blockIdx.x;
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
-  march_team_body<R, M, 0, DS, NR>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<Slot<R> *>(smem), s0, s1);
+  R *const lc = KC > 0 ? reinterpret_cast<R *>(smem + (team_lds_bytes<R>(blockDim.x) + 15) / 16 * 16 +
+                                                 threadIdx.x * coef_cache_lane_bytes<R, M>())
+                       : nullptr;
+  march_team_body<R, M, 0, DS, NR, KC>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<Slot<R> *>(smem), lc, s0,
+                                       s1);
 }
 
 namespace {
@@ -1179,10 +1236,16 @@ constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS of one CU (gfx950)
 
 template <class R, int M, bool DS>
 int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
+  constexpr int KC = DS ? coef_cache_rows<R, M>() : 0;
   if (lanes == 64) {
     const int grid = (a.team_nrays + 3) / 4;
-    const size_t lds = sizeof(R) == 4 ? 4 * static_cast<size_t>(64 + 2 * kPad) * sizeof(Slot<R>) : 0;
-    hipLaunchKernelGGL((k_march_team_w<R, M, DS>), dim3(grid), dim3(256), lds, st, a, s0, s1);
+    const size_t lds = wave_team_lds_bytes<R>();
+    // two blocks per CU (the VGPRs allow two waves per SIMD)
+    const size_t with_cache = lds + 256 * static_cast<size_t>(coef_cache_lane_bytes<R, M>());
+    if (KC > 0 && 2 * with_cache <= kMaxLdsBytes)
+      hipLaunchKernelGGL((k_march_team_w<R, M, DS, KC>), dim3(grid), dim3(256), with_cache, st, a, s0, s1);
+    else
+      hipLaunchKernelGGL((k_march_team_w<R, M, DS, 0>), dim3(grid), dim3(256), lds, st, a, s0, s1);
   } else {
     if (lanes % 64 || lanes > 512) return -1;
     // The LDS rows of the elimination arrays only where their LDS leaves the
@@ -1196,12 +1259,21 @@ int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
     const size_t blocks = waves >= 8 ? 1 : static_cast<size_t>(8 / waves);
     if constexpr (NR < M) {
       if (with_tail * blocks <= kMaxLdsBytes) {
-        hipLaunchKernelGGL((k_march_team_l<R, M, DS, NR>), dim3(a.team_nrays), dim3(lanes), with_tail, st, a, s0, s1);
+        hipLaunchKernelGGL((k_march_team_l<R, M, DS, NR, 0>), dim3(a.team_nrays), dim3(lanes), with_tail, st, a, s0, s1);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+      }
+    }
+    if constexpr (KC > 0) {
+      const size_t with_cache =
+          (base + 15) / 16 * 16 + static_cast<size_t>(lanes) * coef_cache_lane_bytes<R, M>();
+      if (with_cache * blocks <= kMaxLdsBytes) {
+        hipLaunchKernelGGL((k_march_team_l<R, M, DS, M, KC>), dim3(a.team_nrays), dim3(lanes), with_cache, st, a, s0,
+                           s1);
         return hipGetLastError() == hipSuccess ? 0 : -1;
       }
     }
     if (base > kMaxLdsBytes) return -1;
-    hipLaunchKernelGGL((k_march_team_l<R, M, DS, M>), dim3(a.team_nrays), dim3(lanes), base, st, a, s0, s1);
+    hipLaunchKernelGGL((k_march_team_l<R, M, DS, M, 0>), dim3(a.team_nrays), dim3(lanes), base, st, a, s0, s1);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
