@@ -544,6 +544,16 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
   const int sb = wave >> 1, wn = (wave & 1) * 64;
   const int s0 = blockIdx.x * 2, s = s0 + sb;
   const bool live = s < g.S;
+  // the wave's A fragments of hin, loaded before the staging of the modes
+  // below so that both HBM reads are in flight together
+  bf16x8 afr[FC / 16][2];
+  if (live) {
+    const bf16 *hrows = g.hin + static_cast<int64_t>(s) * FZ * FC;
+#pragma unroll
+    for (int kk = 0; kk < FC / 16; ++kk)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) afr[kk][i] = ld8(hrows + (i * 32 + (lane & 31)) * FC + kk * 16 + 8 * (lane >> 5));
+  }
   if (SYN) {
     // stage M[s][k'][c] for both columns: 2 x 32 x 128 bf16 in 16-byte pieces
     for (int q = tid; q < 2 * FK2 * (FC / 8); q += 256) {
@@ -568,7 +578,19 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  if (live) pointwise_tiles(g.hin + static_cast<int64_t>(s) * FZ * FC, g.wt, wn, lane, acc);
+  if (live) {
+#pragma unroll
+    for (int kk = 0; kk < FC / 16; ++kk) {
+      const int k = kk * 16 + 8 * (lane >> 5);
+      bf16x8 bw[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bw[j] = ld8(g.wt + (wn + j * 32 + (lane & 31)) * FC + k);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afr[kk][i], bw[j], acc[i][j], 0, 0, 0);
+    }
+  }
   if (SYN) {
 #pragma unroll
     for (int part = 0; part < 2; ++part) {
