@@ -115,18 +115,37 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
   const bf16 zero = static_cast<bf16>(0.0f);
+  // The 16-byte pieces of op(A) and op(B) a thread stages, for the next K
+  // tile, are loaded while the current tile is multiplied, so the tile's HBM
+  // latency overlaps the MFMA work instead of following the barrier.
+  bf16x8 ra[2], rb[2];
+  auto vec_a = [&](int k) { return g.vec && k + BK <= kend && m0 + BM <= g.M; };
+  auto vec_b = [&](int k) { return g.vec && k + BK <= kend && n0 + BN <= g.N; };
+  auto fetch_a = [&](int k) {
+    const bf16 *src = !TA ? A + static_cast<int64_t>(m0 + (tid >> 1)) * g.lda + k + (tid & 1) * 16
+                          : A + static_cast<int64_t>(k + (tid >> 3)) * g.lda + m0 + (tid & 7) * 16;
+    ra[0] = *reinterpret_cast<const bf16x8 *>(src);
+    ra[1] = *reinterpret_cast<const bf16x8 *>(src + 8);
+  };
+  auto fetch_b = [&](int k) {
+    const bf16 *src = !TB ? B + static_cast<int64_t>(k + (tid >> 3)) * g.ldb + n0 + (tid & 7) * 16
+                          : B + static_cast<int64_t>(n0 + (tid >> 1)) * g.ldb + k + (tid & 1) * 16;
+    rb[0] = *reinterpret_cast<const bf16x8 *>(src);
+    rb[1] = *reinterpret_cast<const bf16x8 *>(src + 8);
+  };
+  bool va = vec_a(kbeg), vb = vec_b(kbeg);
+  if (va) fetch_a(kbeg);
+  if (vb) fetch_b(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    // stage op(A)[m0..m0+127][k0..k0+31] into As[m][k] and op(B) into Bs[n][k];
-    // whole tiles of 16-byte-loadable operands take 16-byte global loads
-    const bool kfull = k0 + BK <= kend;
-    const bool va = g.vec && kfull && m0 + BM <= g.M, vb = g.vec && kfull && n0 + BN <= g.N;
+    // stage op(A)[m0..m0+127][k0..k0+31] into As[m][k] and op(B) into Bs[n][k]:
+    // whole tiles of 16-byte-loadable operands from the prefetched pieces,
+    // the others element by element
     if (!TA) {
       const int r = tid >> 1, kk = (tid & 1) * 16;
       const int m = m0 + r;
       if (va) {
-        const bf16 *src = A + static_cast<int64_t>(m) * g.lda + k0 + kk;
-        *reinterpret_cast<bf16x8 *>(&As[r][kk]) = *reinterpret_cast<const bf16x8 *>(src);
-        *reinterpret_cast<bf16x8 *>(&As[r][kk + 8]) = *reinterpret_cast<const bf16x8 *>(src + 8);
+        *reinterpret_cast<bf16x8 *>(&As[r][kk]) = ra[0];
+        *reinterpret_cast<bf16x8 *>(&As[r][kk + 8]) = ra[1];
       } else {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -138,12 +157,10 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       const int kk = tid >> 3, mm = (tid & 7) * 16;
       const int k = k0 + kk;
       if (va) {
-        const bf16 *src = A + static_cast<int64_t>(k) * g.lda + m0 + mm;
-        const bf16x8 v0 = *reinterpret_cast<const bf16x8 *>(src), v1 = *reinterpret_cast<const bf16x8 *>(src + 8);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-          As[mm + i][kk] = v0[i];
-          As[mm + 8 + i][kk] = v1[i];
+          As[mm + i][kk] = ra[0][i];
+          As[mm + 8 + i][kk] = ra[1][i];
         }
       } else {
 #pragma unroll
@@ -157,12 +174,10 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       const int kk = tid >> 3, nn = (tid & 7) * 16;
       const int k = k0 + kk;
       if (vb) {
-        const bf16 *src = B + static_cast<int64_t>(k) * g.ldb + n0 + nn;
-        const bf16x8 v0 = *reinterpret_cast<const bf16x8 *>(src), v1 = *reinterpret_cast<const bf16x8 *>(src + 8);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-          Bs[nn + i][kk] = v0[i];
-          Bs[nn + 8 + i][kk] = v1[i];
+          Bs[nn + i][kk] = rb[0][i];
+          Bs[nn + 8 + i][kk] = rb[1][i];
         }
       } else {
 #pragma unroll
@@ -175,9 +190,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       const int r = tid >> 1, kk = (tid & 1) * 16;
       const int n = n0 + r;
       if (vb) {
-        const bf16 *src = B + static_cast<int64_t>(n) * g.ldb + k0 + kk;
-        *reinterpret_cast<bf16x8 *>(&Bs[r][kk]) = *reinterpret_cast<const bf16x8 *>(src);
-        *reinterpret_cast<bf16x8 *>(&Bs[r][kk + 8]) = *reinterpret_cast<const bf16x8 *>(src + 8);
+        *reinterpret_cast<bf16x8 *>(&Bs[r][kk]) = rb[0];
+        *reinterpret_cast<bf16x8 *>(&Bs[r][kk + 8]) = rb[1];
       } else {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -187,6 +201,10 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       }
     }
     __syncthreads();
+    const int k1 = k0 + BK;
+    const bool na = k1 < kend && vec_a(k1), nb = k1 < kend && vec_b(k1);
+    if (na) fetch_a(k1);
+    if (nb) fetch_b(k1);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
       bf16x8 af[2], bfr[2];
@@ -202,6 +220,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
+    va = na;
+    vb = nb;
   }
   // epilogue: lane holds column n = lane%32 and rows 8*(r/4) + 4*(lane/32) + r%4
   if (EPI == EPI_BF16 && g.cvec && !g.addend && m0 + BM <= g.M && n0 + BN <= g.N) {
