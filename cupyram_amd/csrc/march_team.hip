@@ -1321,12 +1321,21 @@ void team_auto_shape(int precision, int nz, int *rows, int *lanes) {
     int l = (nz + m - 1) / m;
     l = l <= 64 ? 64 : (l + 63) / 64 * 64;
     if (l > 512) continue;
-    // padded row work, a reduced-system term per lane, a premium for
-    // streamed coefficients, and one for the register spills of lanes past
-    // 24 (complex64) or 12 (complex128) rows, fitted to measured rates
-    const int m_fit = precision == CPR_C64 ? 24 : 12;
-    double prem = team_rows_stream(precision, m) ? 1.15 : 1.0;
-    if (m > m_fit) prem *= std::pow(static_cast<double>(m) / m_fit, 1.2);
+    // padded row work, a reduced-system term per lane, and premiums fitted
+    // to the autotuner's rates: streamed complex128 coefficients (complex64
+    // keeps its streamed rows in LDS), register spills past 24 rows per lane
+    // in complex64 (32 in teams of 192 lanes or more, which keep the last
+    // elimination rows of such lanes in LDS) or 12 in complex128, and teams
+    // of 3 or 5-7 waves, which load the four SIMDs of a CU unevenly (10-25 %
+    // slower than 4 or 8 waves); it picks the tuning table's row count at 36
+    // of its 38 depths
+    const bool c64 = precision == CPR_C64;
+    const int m_fit = c64 ? (l >= 192 ? 32 : 24) : 12;
+    double prem = (!c64 && team_rows_stream(precision, m)) ? 1.15 : 1.0;
+    if (m > m_fit) prem *= std::pow(static_cast<double>(m) / m_fit, c64 ? 3.0 : 1.4);
+    if (!c64 && m > 8) prem *= std::pow(m / 8.0, 0.6);  // complex128 lanes past 8 rows already spill
+    const int waves = l / 64;
+    if (waves == 3 || (waves >= 5 && waves <= 7)) prem *= 1.15;
     const double cost = static_cast<double>(l) * m * prem + 8.0 * l;
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
