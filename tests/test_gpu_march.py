@@ -183,12 +183,14 @@ def test_multi_frequency_matches_single(require_gpu):
                                              ("c128", (40, 64)), ("c64", (20, 64)), ("c64", (21, 64)),
                                              ("c64", (28, 64)), ("c128", (10, 64)), ("c128", (11, 64)),
                                              ("c64", (21, 256)), ("c128", (11, 512)), ("c128", (14, 64)),
-                                             ("c128", (20, 64))])
+                                             ("c128", (20, 64)), ("c64", (21, 128))])
 def test_every_team_shape_matches_oracle(require_gpu, precision, shape):
     """Register coefficients (c64 4..12, c128 2..6), streamed coefficients
     (c64 14..64, c128 8..40; past c64 24 and c128 12 rows the wave teams'
     lanes spill), wave-shuffle (64 lanes) and LDS (>64 lanes) teams, on the
-    wedge (sloping bathymetry moves the water/bottom split)."""
+    wedge (sloping bathymetry moves the water/bottom split).  c64 21 x 128
+    reads its streamed coefficients from the team buffer: its LDS rows would
+    leave the CU three blocks instead of four."""
     tol = 0.1 if precision == "c128" else 0.5
     worst = _compare_with_oracle(envs.wedge(), precision, "team", tol, grids=True, team_shape=shape)
     if precision == "c128":
@@ -199,15 +201,16 @@ def test_every_team_shape_matches_oracle(require_gpu, precision, shape):
 @pytest.mark.parametrize("precision,shape", [("c64", (14, 256)), ("c64", (28, 192)), ("c64", (32, 128)),
                                              ("c64", (40, 256)), ("c64", (64, 192)), ("c128", (12, 320)),
                                              ("c128", (14, 256)), ("c128", (16, 384)), ("c128", (24, 192)),
-                                             ("c128", (40, 128))])
+                                             ("c128", (40, 128)), ("c64", (24, 448))])
 def test_lds_tail_rows_match_oracle(require_gpu, env, precision, shape):
     """LDS teams whose deepest elimination rows live in LDS (complex64 past 24
     rows per lane, complex128 past 11, where that LDS leaves the CU its
     blocks: 28 x 192, 40 x 256, 64 x 192; 12 x 320, 16 x 384, 24 x 192), the
-    same row counts kept in registers (32 x 128; 14 x 256, 40 x 128) and the
-    14-row complex64 lanes, on a 3,535-row Munk column (14 x 256 = 3,584 rows
-    fill the lanes) and on the wedge, whose few valid lanes leave the rest of
-    the team as padding."""
+    same row counts kept in registers (32 x 128; 14 x 256, 40 x 128), the
+    14-row complex64 lanes, and streamed complex64 lanes with their
+    coefficient rows in LDS (24 x 448), on a 3,535-row Munk column
+    (14 x 256 = 3,584 rows fill the lanes) and on the wedge, whose few valid
+    lanes leave the rest of the team as padding."""
     kw = envs.munk_batching(rmax=5000.0) if env == "munk" else envs.wedge(rmax=4000.0)
     tol = 0.1 if precision == "c128" else 0.5
     worst = _compare_with_oracle(kw, precision, "team", tol, grids=env == "wedge", team_shape=shape)
