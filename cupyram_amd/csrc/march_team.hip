@@ -1,6 +1,7 @@
 // Team-of-lanes march for gfx950: the field of a ray stays in registers for
 // every range step of a launch (and, for columns up to 12 rows per lane in
-// complex64 / 6 in complex128, its Galerkin coefficients too).
+// complex64 / 6 in complex128, its Galerkin coefficients too; complex64 lanes
+// of 14-24 rows keep most of theirs in LDS).
 //
 // A team of T lanes owns one ray; lane t holds rows g = 1 + t*M .. (t+1)*M
 // (M rows per lane, a template parameter, so every per-row array is fully
@@ -1216,9 +1217,12 @@ __global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, in
 }
 
 // LDS mode: one team per block of T = blockDim.x lanes (a multiple of 64),
-// with the elimination rows NR .. M-1 in LDS.
+// with the elimination rows NR .. M-1 and the real coefficient halves of rows
+// 0 .. KC-1 in LDS (never both: they start at the same offset).
 template <class R, int M, bool DS, int NR, int KC>
 __global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s1) {
+  // both would start right after the team's buffers
+  static_assert(NR == M || KC == 0, "LDS elimination rows and LDS coefficient rows are not combined");
   extern __shared__ char smem[];
   const int team = blockIdx.x;
   if (team >= a.team_nrays) return;
@@ -1292,8 +1296,8 @@ bool team_rows_stream(int precision, int rows) { return precision == CPR_C64 ? r
 // 21 and 28 (complex64) and 10 and 11 (complex128) let a column fill its
 // lanes with few padding rows (nz = 5303: 21 x 256 = 5376 rows instead of
 // 24 x 256 = 6144), or fill all eight waves of a CU (11 x 512 lanes; complex64
-// 14 x 512 lanes for 7,168 rows: 48.7 against 43.1 G cell-steps/s for
-// 16 x 448).
+// 14 x 512 lanes for 7,168 rows, 50.9 against 44.7 G cell-steps/s for the
+// seven waves of 16 x 448, though 28 x 256 at 58.0 G is the tuned shape).
 constexpr int kRows64[] = {4, 8, 12, 14, 16, 20, 21, 24, 28, 32, 40, 48, 64};
 constexpr int kRows128[] = {2, 4, 6, 8, 10, 11, 12, 14, 16, 20, 24, 32, 40};
 
