@@ -117,6 +117,8 @@ _SYMBOLS = {
     "cpr_fno_project": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_int,
                          C.c_void_p], C.c_int),
     "cpr_fno_layer_project": ([C.c_void_p] * 8 + [C.c_float, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_lift": ([C.c_void_p] * 7 + [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p],
+                     C.c_int),
 }
 
 

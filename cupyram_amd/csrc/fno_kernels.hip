@@ -530,6 +530,17 @@ struct LayerArgs {
   const float *p2;   // [C]
   float c2;
   float *y;          // [S][Z]
+  // LIFT: the first layer formed from the encoding's structure instead of a
+  // product with the encoded input: sample s = query q * nr + range bin r,
+  // h = la[q] + lr[r] + lz[z] + zb[s] lu + (zv[z] > zb[s]) lv
+  const float *la;   // [queries][C] the per-query features' share, plus the bias
+  const float *lr;   // [nr][C] the range features' share
+  const float *lzt;  // [C][Z] the depth features' share (channel-major)
+  const float *lzb;  // [S] local water depth of each sample (encoding units)
+  const float *lzv;  // [Z] depth of each point (encoding units)
+  const float *lu;   // [C] weight of the local water depth
+  const float *lv;   // [C] weight of the below-seabed flag
+  int nr;
   int S;
 };
 
@@ -552,9 +563,10 @@ __device__ inline void pointwise_tiles(const bf16 *hrows, const bf16 *wt, int wn
 
 }  // namespace
 
-template <bool SYN, bool ACT, bool ANA, bool BWD = false, bool PROJ = false>
+template <bool SYN, bool ACT, bool ANA, bool BWD = false, bool PROJ = false, bool LIFT = false>
 __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
   static_assert(!(PROJ && (ANA || BWD)), "the projection follows the last forward layer only");
+  static_assert(!LIFT || (!SYN && !ACT && ANA && !BWD && !PROJ), "the structured lift is the first forward layer");
   // row stride of the [sb][z][c] output tile: padded when the projection
   // reads it back as MFMA rows (16-byte reads of 32 rows at one k)
   constexpr int TL = PROJ ? FC + 8 : FC;
@@ -567,7 +579,7 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
   // the wave's A fragments of hin, loaded before the staging of the modes
   // below so that both HBM reads are in flight together
   bf16x8 afr[FC / 16][2];
-  if (live) {
+  if (!LIFT && live) {
     const bf16 *hrows = g.hin + static_cast<int64_t>(s) * FZ * FC;
 #pragma unroll
     for (int kk = 0; kk < FC / 16; ++kk)
@@ -598,7 +610,33 @@ __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  if (live) {
+  if constexpr (LIFT) {
+    if (live) {
+      // rows z = i*32 + 8*(r/4) + 4*(lane/32) + r%4 of channel n: four
+      // consecutive depths per (i, r/4), one 16-byte read of the
+      // channel-major depth table; the flag from a ballot over the 64 depths
+      const int q = s / g.nr, rr = s - q * g.nr;
+      const float zb = g.lzb[s];
+      const unsigned long long below = __builtin_amdgcn_ballot_w64(g.lzv[lane] > zb);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = wn + j * 32 + (lane & 31);
+        const float base = g.la[static_cast<int64_t>(q) * FC + n] + g.lr[static_cast<int64_t>(rr) * FC + n];
+        const float u = zb * g.lu[n], v = g.lv[n];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int r4 = 0; r4 < 4; ++r4) {
+            const int z0 = i * 32 + 8 * r4 + 4 * (lane >> 5);
+            const float4 t = *reinterpret_cast<const float4 *>(g.lzt + n * FZ + z0);
+            const float tz[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              acc[i][j][4 * r4 + e] = (base + tz[e]) + u + (((below >> (z0 + e)) & 1ull) ? v : 0.0f);
+          }
+      }
+    }
+  } else if (live) {
 #pragma unroll
     for (int kk = 0; kk < FC / 16; ++kk) {
       const int k = kk * 16 + 8 * (lane >> 5);
@@ -1109,6 +1147,32 @@ extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const v
   else
     return cpr_fail("cpr_fno_layer: unsupported (syn, act, ana) combination");
   return launched("cpr_fno_layer");
+}
+
+// The first layer from the encoding's structure (see LayerArgs::la .. lv),
+// with the bias in la, and its truncated DFT: replaces cpr_fno_layer(0, 0, 1,
+// ...) on the encoded input, which reads S x 64 x 128 bf16 of it.
+extern "C" int cpr_fno_lift(const float *la, const float *lr, const float *lzt, const float *lzb,
+                            const float *lzv, const float *lu, const float *lv, int nr, const void *ban, void *hout,
+                            void *xh, int S, void *stream) {
+  if (S <= 0 || nr <= 0 || S % nr || !la || !lr || !lzt || !lzb || !lzv || !lu || !lv || !ban || !hout || !xh)
+    return cpr_fail("cpr_fno_lift: invalid arguments");
+  LayerArgs g{};
+  g.la = la;
+  g.lr = lr;
+  g.lzt = lzt;
+  g.lzb = lzb;
+  g.lzv = lzv;
+  g.lu = lu;
+  g.lv = lv;
+  g.nr = nr;
+  g.ban = static_cast<const bf16 *>(ban);
+  g.hout = static_cast<bf16 *>(hout);
+  g.xh = static_cast<bf16 *>(xh);
+  g.S = S;
+  hipLaunchKernelGGL((k_fno_layer<false, false, true, false, false, true>), dim3((S + 1) / 2), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), g);
+  return launched("cpr_fno_lift");
 }
 
 extern "C" int cpr_fno_layer_project(const void *hin, const void *wt, const float *bias, const void *mh,

@@ -149,6 +149,87 @@ def encode(dom: Domain, q: Queries, features: int = 128, device="cuda"):
     return torch.cat(parts, dim=-1).reshape(n * nr, nz, features)
 
 
+# Rows of the first layer's weight for each column of `encode`: the 101
+# query numbers, then the local features, then the Fourier features
+F_RANGE, F_DEPTH, F_ZB, F_HEIGHT, F_ZREL, F_BELOW, F_ONE = range(N_GLOBAL, N_GLOBAL + N_LOCAL)
+F_LOGR = N_GLOBAL + N_LOCAL
+F_SIN_R = [F_LOGR + 1 + 4 * i for i in range(len(FOURIER_K))]
+F_COS_R = [f + 1 for f in F_SIN_R]
+F_SIN_Z = [f + 2 for f in F_SIN_R]
+F_COS_Z = [f + 3 for f in F_SIN_R]
+
+
+@dataclasses.dataclass
+class LiftInputs:
+    """What the first layer needs of `encode`'s columns, at their natural
+    sizes (float32): the first layer of the network is linear in them, so it
+    is formed from per-query, per-range-bin and per-depth-point tables
+    (`FNO.forward_fused_lift`) instead of a product with the encoded input.
+
+    g       [n, 101]  the normalised query numbers
+    zb      [n * nr]  local water depth of each sample (sample = query, range bin)
+    zv      [nzp]     depth of each point (the depth column)
+    fr      [nr, 10]  range, log range, and sin, cos of pi k r/rmax
+    fr_rows           the first layer's rows of those columns
+    fz      [nzp, 12] depth as it enters the depth, height-above-seabed and
+                      offset-from-source columns, the constant, and sin, cos
+                      of pi k z/zplot
+    fz_rows           the first layer's rows of those columns
+    The query's water depth enters the water-depth column and, negated, the
+    height column; its source depth enters the offset column negated; the
+    below-seabed flag is zv > zb."""
+    g: torch.Tensor
+    zb: torch.Tensor
+    zv: torch.Tensor
+    fr: torch.Tensor
+    fr_rows: list
+    fz: torch.Tensor
+    fz_rows: list
+    nr: int
+
+
+_DOMAIN_TABLES = {}
+
+
+def _domain_tables(dom: Domain, device):
+    """(r, z, fr, fr_rows, fz, fz_rows) of a domain, built once per device."""
+    key = (dom, str(device))
+    if key in _DOMAIN_TABLES:
+        return _DOMAIN_TABLES[key]
+    dev = torch.device(device)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev)  # noqa: E731
+    r = t(dom.ranges() / dom.rmax)
+    z = t(dom.depths() / dom.zplot)
+    fr = [r, torch.log(r)]
+    fr_rows = [F_RANGE, F_LOGR]
+    fz = [z, z, z, torch.ones_like(z)]
+    fz_rows = [F_DEPTH, F_HEIGHT, F_ZREL, F_ONE]
+    for i, kf in enumerate(FOURIER_K):
+        fr += [torch.sin(np.pi * kf * r), torch.cos(np.pi * kf * r)]
+        fr_rows += [F_SIN_R[i], F_COS_R[i]]
+        fz += [torch.sin(np.pi * kf * z), torch.cos(np.pi * kf * z)]
+        fz_rows += [F_SIN_Z[i], F_COS_Z[i]]
+    tables = (r, z.contiguous(), torch.stack(fr, dim=-1).contiguous(), fr_rows,
+              torch.stack(fz, dim=-1).contiguous(), fz_rows)
+    _DOMAIN_TABLES[key] = tables
+    return tables
+
+
+def lift_inputs(dom: Domain, q: Queries, device="cuda") -> LiftInputs:
+    """The tables of `LiftInputs` for the queries `q`, computed exactly as
+    `encode` computes the columns (before its bf16 rounding); the range and
+    depth tables are the domain's, shared by every call."""
+    r, z, fr, fr_rows, fz, fz_rows = _domain_tables(dom, device)
+    g = global_features(dom, q, device)
+    pos = r * (N_BATHY - 1)
+    i0 = torch.clamp(pos.floor().long(), 0, N_BATHY - 2)
+    w = (pos - i0.float())[None, :]
+    bz = g[:, N_SSP:N_SSP + N_BATHY]
+    zb = bz[:, i0] * (1 - w) + bz[:, i0 + 1] * w
+    return LiftInputs(g=g.contiguous(), zb=zb.reshape(-1).contiguous(), zv=z, fr=fr, fr_rows=fr_rows, fz=fz,
+                      fz_rows=fz_rows, nr=dom.nr)
+
+
 def normalise_tl(dom: Domain, tl):
     """(target, mask) from TL [.., nr, nzp] (dB): clipped, centred, scaled;
     non-finite points are masked out."""

@@ -134,6 +134,7 @@ class FNO:
         for n, t in self.bf.items():
             k.cast_bf16(self.params[n], t)
         self._fused = None  # output bias of the fused projection, read again on next use
+        self._lift = None   # tables of the structured first layer, rebuilt on next use
         self._wt = None     # transposed weights of the fused kernels, rebuilt on next use
         self._expand()
 
@@ -264,19 +265,73 @@ class FNO:
         if Z != cfg.depth_points or F != cfg.features or x.dtype != torch.bfloat16:
             raise ValueError(f"input must be bf16 [S, {cfg.depth_points}, {cfg.features}]")
         x = x.contiguous()
-        bf = torch.bfloat16
         if self._fused is None or self._fused[0] != self.step_count:
             # the output bias as a host float
             self._fused = (self.step_count, float(self.params["c2"].item()))
         wt, c2 = self._transposed(), self._fused[1]
-        Cw, Km, L = cfg.width, cfg.modes, cfg.layers
+        Cw, Km = cfg.width, cfg.modes
         R = S * Z
         bf = torch.bfloat16
         h = [self._buf("fh0", (R, Cw), bf), self._buf("fh1", (R, Cw), bf)]
         xh = self._buf("fxh", (Km, S, 2 * Cw), bf)
+        k.fno_layer(x, wt["W0"], self.params["b0"], h[0], S, ban=self.ban, xh=xh, act=False)
+        return self._fused_layers(S, h, xh, wt, c2)
+
+    def lift_tables(self, inp):
+        """(la, lr, lzt, lu, lv) of `kernels.fno_lift` for `inp`
+        (`encode.LiftInputs`): the first layer is linear in the encoding's
+        columns, so its output is the per-query columns' share (with the
+        bias), the range bin's, the depth point's, the local water depth's and
+        the below-seabed flag's; the weights are the bf16 ones the product
+        with the encoded input uses.  All but la depend only on the weights
+        and the domain's tables and are kept until either changes."""
+        from .encode import F_BELOW, F_HEIGHT, F_ZB, F_ZREL, N_GLOBAL
+        c = self._lift
+        if c is None or c[0] != self.step_count or c[1] is not inp.fr or c[2] is not inp.fz:
+            W = self.bf["W0"].float()                                 # [F, C]
+            wg = W[:N_GLOBAL].clone()
+            wg[N_GLOBAL - 1] -= W[F_ZREL]    # the offset column holds z - zs
+            lr = (inp.fr @ W[inp.fr_rows]).contiguous()
+            lzt = (inp.fz @ W[inp.fz_rows]).t().contiguous()
+            lu = (W[F_ZB] - W[F_HEIGHT]).contiguous()    # the height column holds z - zb
+            lv = W[F_BELOW].contiguous()
+            c = self._lift = (self.step_count, inp.fr, inp.fz, wg, lr, lzt, lu, lv)
+        la = torch.addmm(self.params["b0"], inp.g, c[3])
+        return (la,) + c[4:]
+
+    def forward_fused_lift(self, inp):
+        """`forward_fused` of the encoded queries without the encoded input:
+        the first layer from `inp` (`encode.LiftInputs`) in one kernel
+        (`kernels.fno_lift`), the rest as `forward_fused`.  Equal to it up to
+        the bf16 rounding of the encoded columns, which this form skips."""
+        cfg = self.cfg
+        if not self.fusable:
+            raise ValueError("the fused kernels are built for Z=64, width=128, modes=16, features=128")
+        n, nr = inp.g.shape[0], inp.nr
+        S = n * nr
+        if inp.zv.numel() != cfg.depth_points:
+            raise ValueError(f"the lift inputs must have {cfg.depth_points} depth points")
+        if self._fused is None or self._fused[0] != self.step_count:
+            self._fused = (self.step_count, float(self.params["c2"].item()))
+        wt, c2 = self._transposed(), self._fused[1]
+        Cw, Km = cfg.width, cfg.modes
+        R = S * cfg.depth_points
+        bf = torch.bfloat16
+        h = [self._buf("fh0", (R, Cw), bf), self._buf("fh1", (R, Cw), bf)]
+        xh = self._buf("fxh", (Km, S, 2 * Cw), bf)
+        la, lr, lzt, lu, lv = self.lift_tables(inp)
+        k.fno_lift(la, lr, lzt, inp.zb, inp.zv, lu, lv, nr, self.ban, h[0], xh, S)
+        return self._fused_layers(S, h, xh, wt, c2)
+
+    def _fused_layers(self, S, h, xh, wt, c2):
+        """Layers 1 .. L and the projection of the fused inference, from the
+        first layer's output h[0] and its modes xh."""
+        cfg = self.cfg
+        Cw, Km, L = cfg.width, cfg.modes, cfg.layers
+        R = S * cfg.depth_points
+        bf = torch.bfloat16
         mh = self._buf("fmh", (Km, S, 2 * Cw), bf)
         pred = self._buf("fpred", (R,), torch.float32)
-        k.fno_layer(x, wt["W0"], self.params["b0"], h[0], S, ban=self.ban, xh=xh, act=False)
         for l in range(1, L + 1):
             k.gemm(xh, self.blk_t[l], mh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
                    sc=S * 2 * Cw, epi=k.EPI_BF16)

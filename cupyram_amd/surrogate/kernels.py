@@ -247,6 +247,27 @@ def fno_layer_project(hin, wt, bias, mh, bsyn, p1t, c1, p2, c2, y, S):
                                                 _p(p2), C.c_float(float(c2)), _p(y), int(S), _stream()))
 
 
+def fno_lift(la, lr, lzt, lzb, lzv, lu, lv, nr, ban, hout, xh, S):
+    """The first layer from the encoding's structure (cpr_fno_lift): for
+    sample s = q * nr + r and depth z, hout = la[q] + lr[r] + lzt[:, z] +
+    lzb[s] lu + (lzv[z] > lzb[s]) lv, and xh its truncated DFT."""
+    Z, Cw, K2 = FUSED_Z, FUSED_C, 2 * FUSED_K
+    if S % nr:
+        raise ValueError("fno_lift: the samples are whole queries of nr range bins")
+    _need(la, torch.float32, (S // nr) * Cw, "fno_lift la")
+    _need(lr, torch.float32, nr * Cw, "fno_lift lr")
+    _need(lzt, torch.float32, Cw * Z, "fno_lift lzt")
+    _need(lzb, torch.float32, S, "fno_lift lzb")
+    _need(lzv, torch.float32, Z, "fno_lift lzv")
+    _need(lu, torch.float32, Cw, "fno_lift lu")
+    _need(lv, torch.float32, Cw, "fno_lift lv")
+    _need(ban, torch.bfloat16, 2 * K2 * Z, "fno_lift ban")
+    _need(hout, torch.bfloat16, S * Z * Cw, "fno_lift hout")
+    _need(xh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_lift xh")
+    _abi.check(_abi.lib().cpr_fno_lift(_p(la), _p(lr), _p(lzt), _p(lzb), _p(lzv), _p(lu), _p(lv), int(nr), _p(ban),
+                                       _p(hout), _p(xh), int(S), _stream()))
+
+
 def cast_bf16_t(x, y):
     """y [C][R] bf16 = x [R][C] f32 transposed."""
     R, Cc = x.shape

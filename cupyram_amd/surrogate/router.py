@@ -22,7 +22,7 @@ import time
 import numpy as np
 import torch
 
-from .encode import Domain, Queries, denormalise_tl, encode
+from .encode import Domain, Queries, denormalise_tl, encode, lift_inputs
 from .fno import infer
 from .train import field_rms, pe_fields
 
@@ -70,13 +70,19 @@ class Router:
         mean = torch.empty(n, dom.nr, dom.nzp, device=dev)
         spread = torch.empty(n, device=dev)
         feats = self.members[0].cfg.features
+        # members with the fused kernels form their first layer from the
+        # encoding's tables (forward_fused_lift); the others read the encoded input
+        lifted = [bool(getattr(m, "fusable", False)) for m in self.members]
         for a in range(0, n, self.batch):
             idx = np.arange(a, min(n, a + self.batch))
-            x = encode(dom, q.take(idx), feats, device=dev)
+            sub = q.take(idx)
+            inp = lift_inputs(dom, sub, device=dev) if any(lifted) else None
+            x = encode(dom, sub, feats, device=dev) if not all(lifted) else None
             s1 = torch.zeros(len(idx), dom.nr, dom.nzp, device=dev)
             s2 = torch.zeros_like(s1)
-            for m in self.members:
-                y = denormalise_tl(dom, infer(m, x).view(len(idx), dom.nr, dom.nzp))
+            for m, lift in zip(self.members, lifted):
+                p = m.forward_fused_lift(inp) if lift else infer(m, x)
+                y = denormalise_tl(dom, p.view(len(idx), dom.nr, dom.nzp))
                 y = y.clamp(dom.tl_lo, dom.tl_hi)
                 s1 += y
                 s2 += y * y

@@ -252,6 +252,18 @@ int cpr_fno_project(const void *h, const void *p1t, const float *c1, const float
 int cpr_fno_layer_project(const void *hin, const void *wt, const float *bias, const void *mh, const void *bsyn,
                           const void *p1t, const float *c1, const float *p2, float c2, float *y, int S,
                           void *stream);
+/* The first layer (cpr_fno_layer with syn = act = 0, ana = 1) formed from
+ * the input encoding's structure instead of a product with the encoded
+ * input: sample s = q * nr + r (query q, range bin r), depth point z,
+ * hout[s][z][c] = la[q][c] + lr[r][c] + lzt[c][z] + lzb[s] lu[c]
+ * + (lzv[z] > lzb[s] ? lv[c] : 0), all f32 (la holds the per-query
+ * features' share and the bias, lr the range features', lzt the depth
+ * features' (channel-major, 16-byte aligned), lzb the local water depth,
+ * lzv the depth of each point), and xh its truncated DFT as cpr_fno_layer
+ * writes it.  S must be a multiple of nr (inference). */
+int cpr_fno_lift(const float *la, const float *lr, const float *lzt, const float *lzb, const float *lzv,
+                 const float *lu, const float *lv, int nr, const void *ban, void *hout, void *xh, int S,
+                 void *stream);
 int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *stream);
 /* y [Cc][R] bf16 = x [R][Cc] f32 transposed. */
 int cpr_fno_cast_bf16_t(const float *x, void *y, int R, int Cc, void *stream);

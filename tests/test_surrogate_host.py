@@ -143,3 +143,34 @@ def test_encode_equals_feature_by_feature_form(name):
     q = gen(4, 9)
     assert torch.equal(encode(dom, q, device="cpu"), _encode_by_slices(dom, q))
 
+
+
+@pytest.mark.parametrize("name", ["wedge", "deep_munk"])
+def test_lift_inputs_reproduce_encode(name):
+    """The tables the structured first layer reads (lift_inputs) hold the
+    encoded columns: rebuilt column by column from them as their docstring
+    states (query numbers; range and depth tables; the water depth in its
+    own column, subtracted in the height column; the source depth subtracted
+    in the offset column; the flag zv > zb) and cast once, they equal
+    encode() bit for bit."""
+    torch = pytest.importorskip("torch")
+    import importlib
+    E = importlib.import_module("cupyram_amd.surrogate.encode")
+    dom = D.DOMAINS[name]
+    gen = D.wedge_queries if name == "wedge" else D.deep_queries
+    q = gen(3, 11)
+    inp = E.lift_inputs(dom, q, device="cpu")
+    n, nr, nz = len(q), dom.nr, dom.nzp
+    zb = inp.zb.reshape(n, nr)
+    out = torch.zeros(n, nr, nz, 128)
+    out[..., :E.N_GLOBAL] = inp.g[:, None, None, :]
+    for j, row in enumerate(inp.fr_rows):
+        out[..., row] = inp.fr[None, :, None, j]
+    for j, row in enumerate(inp.fz_rows):
+        out[..., row] = inp.fz[None, None, :, j]
+    out[..., E.F_HEIGHT] = out[..., E.F_HEIGHT] - zb[:, :, None]
+    out[..., E.F_ZREL] = out[..., E.F_ZREL] - inp.g[:, None, None, E.N_GLOBAL - 1]
+    out[..., E.F_ZB] = zb[:, :, None]
+    out[..., E.F_BELOW] = (inp.zv[None, None, :] > zb[:, :, None]).float()
+    assert sorted(inp.fr_rows + inp.fz_rows + [E.F_ZB, E.F_BELOW]) == list(range(E.N_GLOBAL, 125))
+    assert torch.equal(out.to(torch.bfloat16).reshape(n * nr, nz, 128), encode(dom, q, device="cpu"))
