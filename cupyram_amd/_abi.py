@@ -119,6 +119,8 @@ _SYMBOLS = {
     "cpr_fno_layer_project": ([C.c_void_p] * 8 + [C.c_float, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "cpr_fno_lift": ([C.c_void_p] * 7 + [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p],
                      C.c_int),
+    "cpr_fno_lift_modes": ([C.c_void_p] * 8 + [C.c_int, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "cpr_fno_layer_lift": ([C.c_void_p] * 7 + [C.c_int] + [C.c_void_p] * 6 + [C.c_int, C.c_void_p], C.c_int),
 }
 
 

@@ -566,7 +566,8 @@ __device__ inline void pointwise_tiles(const bf16 *hrows, const bf16 *wt, int wn
 template <bool SYN, bool ACT, bool ANA, bool BWD = false, bool PROJ = false, bool LIFT = false>
 __global__ void __launch_bounds__(256) k_fno_layer(LayerArgs g) {
   static_assert(!(PROJ && (ANA || BWD)), "the projection follows the last forward layer only");
-  static_assert(!LIFT || (!SYN && !ACT && ANA && !BWD && !PROJ), "the structured lift is the first forward layer");
+  static_assert(!LIFT || (ANA && !BWD && !PROJ && SYN == ACT),
+                "the table form of the pointwise product: the first forward layer, or the second one");
   // row stride of the [sb][z][c] output tile: padded when the projection
   // reads it back as MFMA rows (16-byte reads of 32 rows at one k)
   constexpr int TL = PROJ ? FC + 8 : FC;
@@ -1149,6 +1150,31 @@ extern "C" int cpr_fno_layer(int syn, int act, int ana, const void *hin, const v
   return launched("cpr_fno_layer");
 }
 
+// Modes of the first layer in the table form, without the layer's output:
+// with h[s][z][c] = A[s][c] + lz[z][c] + (z >= cut(s)) lv[c] (A = la[q] +
+// lr[r] + zb[s] lu, cut(s) the number of depth points at or above the local
+// water depth), the truncated DFT sum_z An[k'][z] h[s][z][c] is
+// A sig[k'] + bz[k'][c] + lv[c] tsum[k'][cut(s)], where sig[k'] = tsum[k'][0],
+// tsum[k'][m] = sum_{z >= m} An[k'][z] and bz = An lz (the analysis matrix An
+// in FP32).  One block of 128 threads (one per channel) per sample; xh as
+// cpr_fno_layer writes it (bf16 [K][S][2C], real parts then imaginary).
+__global__ void __launch_bounds__(128) k_lift_modes(const float *la, const float *lr, const float *lzb,
+                                                     const float *lzv, const float *lu, const float *lv,
+                                                     const float *tsum, const float *bz, int nr, bf16 *xh, int S) {
+  const int s = blockIdx.x, c = threadIdx.x;
+  const float zb = lzb[s];
+  const int cut = __syncthreads_count(c < FZ && lzv[c < FZ ? c : 0] <= zb);
+  const int q = s / nr, r = s - q * nr;
+  const float a = la[static_cast<int64_t>(q) * FC + c] + lr[static_cast<int64_t>(r) * FC + c] + zb * lu[c];
+  const float v = lv[c];
+#pragma unroll 4
+  for (int kp = 0; kp < FK2; ++kp) {
+    const float x = a * tsum[kp * (FZ + 1)] + bz[kp * FC + c] + v * tsum[kp * (FZ + 1) + cut];
+    const int k = kp % (FK2 / 2), im = kp / (FK2 / 2);
+    xh[(static_cast<int64_t>(k) * S + s) * 2 * FC + im * FC + c] = static_cast<bf16>(x);
+  }
+}
+
 // The first layer from the encoding's structure (see LayerArgs::la .. lv),
 // with the bias in la, and its truncated DFT: replaces cpr_fno_layer(0, 0, 1,
 // ...) on the encoded input, which reads S x 64 x 128 bf16 of it.
@@ -1173,6 +1199,49 @@ extern "C" int cpr_fno_lift(const float *la, const float *lr, const float *lzt, 
   hipLaunchKernelGGL((k_fno_layer<false, false, true, false, false, true>), dim3((S + 1) / 2), dim3(256), 0,
                      static_cast<hipStream_t>(stream), g);
   return launched("cpr_fno_lift");
+}
+
+// The first layer's modes in the table form (k_lift_modes); tsum [2K][65],
+// bz [2K][128] f32 from the analysis matrix and the depth features' share.
+extern "C" int cpr_fno_lift_modes(const float *la, const float *lr, const float *lzb, const float *lzv,
+                                  const float *lu, const float *lv, const float *tsum, const float *bz, int nr,
+                                  void *xh, int S, void *stream) {
+  if (S <= 0 || nr <= 0 || S % nr || !la || !lr || !lzb || !lzv || !lu || !lv || !tsum || !bz || !xh)
+    return cpr_fail("cpr_fno_lift_modes: invalid arguments");
+  hipLaunchKernelGGL(k_lift_modes, dim3(S), dim3(FC), 0, static_cast<hipStream_t>(stream), la, lr, lzb, lzv, lu, lv,
+                     tsum, bz, nr, static_cast<bf16 *>(xh), S);
+  return launched("cpr_fno_lift_modes");
+}
+
+// The second layer with its pointwise product in the table form (the first
+// layer's tables multiplied by the layer's weights; bias separate), the
+// spectral synthesis of mh, GELU and the next analysis, as cpr_fno_layer.
+extern "C" int cpr_fno_layer_lift(const float *la, const float *lr, const float *lzt, const float *lzb,
+                                  const float *lzv, const float *lu, const float *lv, int nr, const float *bias,
+                                  const void *mh, const void *bsyn, const void *ban, void *hout, void *xh, int S,
+                                  void *stream) {
+  if (S <= 0 || nr <= 0 || S % nr || !la || !lr || !lzt || !lzb || !lzv || !lu || !lv || !mh || !bsyn || !ban ||
+      !hout || !xh)
+    return cpr_fail("cpr_fno_layer_lift: invalid arguments");
+  LayerArgs g{};
+  g.la = la;
+  g.lr = lr;
+  g.lzt = lzt;
+  g.lzb = lzb;
+  g.lzv = lzv;
+  g.lu = lu;
+  g.lv = lv;
+  g.nr = nr;
+  g.bias = bias;
+  g.mh = static_cast<const bf16 *>(mh);
+  g.bsyn = static_cast<const bf16 *>(bsyn);
+  g.ban = static_cast<const bf16 *>(ban);
+  g.hout = static_cast<bf16 *>(hout);
+  g.xh = static_cast<bf16 *>(xh);
+  g.S = S;
+  hipLaunchKernelGGL((k_fno_layer<true, true, true, false, false, true>), dim3((S + 1) / 2), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), g);
+  return launched("cpr_fno_layer_lift");
 }
 
 extern "C" int cpr_fno_layer_project(const void *hin, const void *wt, const float *bias, const void *mh,

@@ -96,6 +96,8 @@ class FNO:
             ana = np.concatenate([ac, as_], axis=0)           # [2K][Z]
             self.bsyn = self._hi_lo(syn)
             self.ban = self._hi_lo(ana)
+            # FP32 analysis matrix: the first layer's modes in the table form
+            self.ana = to(ana)
             # the backward pass applies the transposes: the analysis matrix
             # transposed as its synthesis ([Z][2K]) and the synthesis matrix
             # transposed as its analysis ([2K][Z])
@@ -297,13 +299,17 @@ class FNO:
             lv = W[F_BELOW].contiguous()
             c = self._lift = (self.step_count, inp.fr, inp.fz, wg, lr, lzt, lu, lv)
         la = torch.addmm(self.params["b0"], inp.g, c[3])
-        return (la,) + c[4:]
+        return (la,) + c[4:8]
 
     def forward_fused_lift(self, inp):
-        """`forward_fused` of the encoded queries without the encoded input:
-        the first layer from `inp` (`encode.LiftInputs`) in one kernel
-        (`kernels.fno_lift`), the rest as `forward_fused`.  Equal to it up to
-        the bf16 rounding of the encoded columns, which this form skips."""
+        """`forward_fused` of the encoded queries without the encoded input
+        and without the first layer's output: the first layer's modes from
+        its tables (`kernels.fno_lift_modes`), the second layer's pointwise
+        product from the same tables times its weight
+        (`kernels.fno_layer_lift`), the rest as `forward_fused`.  Equal to it
+        up to the bf16 rounding of the encoded columns and of the first
+        layer's output, which this form skips.  A one-layer network takes the
+        first layer's output from `kernels.fno_lift` instead."""
         cfg = self.cfg
         if not self.fusable:
             raise ValueError("the fused kernels are built for Z=64, width=128, modes=16, features=128")
@@ -320,19 +326,49 @@ class FNO:
         h = [self._buf("fh0", (R, Cw), bf), self._buf("fh1", (R, Cw), bf)]
         xh = self._buf("fxh", (Km, S, 2 * Cw), bf)
         la, lr, lzt, lu, lv = self.lift_tables(inp)
-        k.fno_lift(la, lr, lzt, inp.zb, inp.zv, lu, lv, nr, self.ban, h[0], xh, S)
-        return self._fused_layers(S, h, xh, wt, c2)
+        if cfg.layers < 2:
+            k.fno_lift(la, lr, lzt, inp.zb, inp.zv, lu, lv, nr, self.ban, h[0], xh, S)
+            return self._fused_layers(S, h, xh, wt, c2)
+        tsum, bz, lr1, lzt1, lu1, lv1, w1 = self._lift_second_tables()
+        k.fno_lift_modes(la, lr, inp.zb, inp.zv, lu, lv, tsum, bz, nr, xh, S)
+        mh = self._buf("fmh", (Km, S, 2 * Cw), bf)
+        k.gemm(xh, self.blk_t[1], mh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
+               sc=S * 2 * Cw, epi=k.EPI_BF16)
+        la1 = la @ w1
+        k.fno_layer_lift(la1, lr1, lzt1, inp.zb, inp.zv, lu1, lv1, nr, self.params["bp1"], mh, self.bsyn, self.ban,
+                         h[1], xh, S)
+        return self._fused_layers(S, h, xh, wt, c2, first=2)
 
-    def _fused_layers(self, S, h, xh, wt, c2):
-        """Layers 1 .. L and the projection of the fused inference, from the
-        first layer's output h[0] and its modes xh."""
+    def _lift_second_tables(self):
+        """Tables of the first layer's modes and of the second layer's
+        pointwise product in the table form, from the cached first-layer
+        tables (call after `lift_tables`): the analysis rows' suffix sums over
+        depth [2K][Z+1], the analysis of the depth features' share [2K][C],
+        and the range, depth, water-depth and flag tables times the second
+        layer's bf16 weight, with that weight [C][C] (for la)."""
+        c = self._lift
+        if len(c) == 8:
+            lr, lzt, lu, lv = c[4:8]
+            w1 = self.bf["Wp1"].float()                                # [C in, C out]
+            an = self.ana                                             # [2K][Z]
+            tsum = torch.zeros(an.shape[0], an.shape[1] + 1, device=self.device)
+            tsum[:, :-1] = torch.flip(torch.cumsum(torch.flip(an, [1]), 1), [1])
+            bz = an @ lzt.t()
+            extra = (tsum.contiguous(), bz.contiguous(), (lr @ w1).contiguous(), (lzt.t() @ w1).t().contiguous(),
+                     (lu @ w1).contiguous(), (lv @ w1).contiguous(), w1)
+            c = self._lift = c + extra
+        return c[8:]
+
+    def _fused_layers(self, S, h, xh, wt, c2, first=1):
+        """Layers first .. L and the projection of the fused inference, from
+        the output of layer first - 1 in h[(first - 1) % 2] and its modes xh."""
         cfg = self.cfg
         Cw, Km, L = cfg.width, cfg.modes, cfg.layers
         R = S * cfg.depth_points
         bf = torch.bfloat16
         mh = self._buf("fmh", (Km, S, 2 * Cw), bf)
         pred = self._buf("fpred", (R,), torch.float32)
-        for l in range(1, L + 1):
+        for l in range(first, L + 1):
             k.gemm(xh, self.blk_t[l], mh, S, 2 * Cw, 2 * Cw, tb=True, batch=Km, sa=S * 2 * Cw, sb=4 * Cw * Cw,
                    sc=S * 2 * Cw, epi=k.EPI_BF16)
             if l < L:

@@ -268,6 +268,52 @@ def fno_lift(la, lr, lzt, lzb, lzv, lu, lv, nr, ban, hout, xh, S):
                                        _p(hout), _p(xh), int(S), _stream()))
 
 
+def fno_lift_modes(la, lr, lzb, lzv, lu, lv, tsum, bz, nr, xh, S):
+    """The first layer's modes in the table form without the layer's output
+    (cpr_fno_lift_modes): xh = la[q] + lr[r] + lzb[s] lu times the analysis
+    rows' sums, plus bz, plus lv times the analysis rows' sums over the
+    depths below the local water depth (tsum [2K][Z+1] suffix sums)."""
+    Z, Cw, K2 = FUSED_Z, FUSED_C, 2 * FUSED_K
+    if S % nr:
+        raise ValueError("fno_lift_modes: the samples are whole queries of nr range bins")
+    _need(la, torch.float32, (S // nr) * Cw, "fno_lift_modes la")
+    _need(lr, torch.float32, nr * Cw, "fno_lift_modes lr")
+    _need(lzb, torch.float32, S, "fno_lift_modes lzb")
+    _need(lzv, torch.float32, Z, "fno_lift_modes lzv")
+    _need(lu, torch.float32, Cw, "fno_lift_modes lu")
+    _need(lv, torch.float32, Cw, "fno_lift_modes lv")
+    _need(tsum, torch.float32, K2 * (Z + 1), "fno_lift_modes tsum")
+    _need(bz, torch.float32, K2 * Cw, "fno_lift_modes bz")
+    _need(xh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_lift_modes xh")
+    _abi.check(_abi.lib().cpr_fno_lift_modes(_p(la), _p(lr), _p(lzb), _p(lzv), _p(lu), _p(lv), _p(tsum), _p(bz),
+                                             int(nr), _p(xh), int(S), _stream()))
+
+
+def fno_layer_lift(la, lr, lzt, lzb, lzv, lu, lv, nr, bias, mh, bsyn, ban, hout, xh, S):
+    """The second layer with its pointwise product in the table form
+    (cpr_fno_layer_lift): hout = GELU(la[q] + lr[r] + lzt[:, z] + lzb[s] lu +
+    (lzv[z] > lzb[s]) lv + bias + syn(mh)), xh its truncated DFT."""
+    Z, Cw, K2 = FUSED_Z, FUSED_C, 2 * FUSED_K
+    if S % nr:
+        raise ValueError("fno_layer_lift: the samples are whole queries of nr range bins")
+    _need(la, torch.float32, (S // nr) * Cw, "fno_layer_lift la")
+    _need(lr, torch.float32, nr * Cw, "fno_layer_lift lr")
+    _need(lzt, torch.float32, Cw * Z, "fno_layer_lift lzt")
+    _need(lzb, torch.float32, S, "fno_layer_lift lzb")
+    _need(lzv, torch.float32, Z, "fno_layer_lift lzv")
+    _need(lu, torch.float32, Cw, "fno_layer_lift lu")
+    _need(lv, torch.float32, Cw, "fno_layer_lift lv")
+    _need(bias, torch.float32, Cw, "fno_layer_lift bias")
+    _need(mh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer_lift mh")
+    _need(bsyn, torch.bfloat16, 2 * Z * K2, "fno_layer_lift bsyn")
+    _need(ban, torch.bfloat16, 2 * K2 * Z, "fno_layer_lift ban")
+    _need(hout, torch.bfloat16, S * Z * Cw, "fno_layer_lift hout")
+    _need(xh, torch.bfloat16, FUSED_K * S * 2 * Cw, "fno_layer_lift xh")
+    _abi.check(_abi.lib().cpr_fno_layer_lift(_p(la), _p(lr), _p(lzt), _p(lzb), _p(lzv), _p(lu), _p(lv), int(nr),
+                                             _p(bias), _p(mh), _p(bsyn), _p(ban), _p(hout), _p(xh), int(S),
+                                             _stream()))
+
+
 def cast_bf16_t(x, y):
     """y [C][R] bf16 = x [R][C] f32 transposed."""
     R, Cc = x.shape

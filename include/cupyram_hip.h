@@ -264,6 +264,21 @@ int cpr_fno_layer_project(const void *hin, const void *wt, const float *bias, co
 int cpr_fno_lift(const float *la, const float *lr, const float *lzt, const float *lzb, const float *lzv,
                  const float *lu, const float *lv, int nr, const void *ban, void *hout, void *xh, int S,
                  void *stream);
+/* The first layer's modes xh in the table form of cpr_fno_lift, without the
+ * layer's output: A = la[q] + lr[r] + lzb[s] lu, xh[k'] = A tsum[k'][0] +
+ * bz[k'] + lv tsum[k'][cut(s)] with tsum [2K][65] f32 the suffix sums over
+ * depth of the analysis rows (tsum[k'][m] = sum_{z >= m} An[k'][z]), bz
+ * [2K][128] f32 the analysis of the depth features' share, cut(s) the number
+ * of depth points with lzv <= lzb[s]. */
+int cpr_fno_lift_modes(const float *la, const float *lr, const float *lzb, const float *lzv, const float *lu,
+                       const float *lv, const float *tsum, const float *bz, int nr, void *xh, int S, void *stream);
+/* The second layer (cpr_fno_layer with syn = act = ana = 1) whose pointwise
+ * product of the first layer's output is formed from the first layer's
+ * tables times its weight (the same table form as cpr_fno_lift, bias
+ * separate), so the first layer's output is never written (inference). */
+int cpr_fno_layer_lift(const float *la, const float *lr, const float *lzt, const float *lzb, const float *lzv,
+                       const float *lu, const float *lv, int nr, const float *bias, const void *mh,
+                       const void *bsyn, const void *ban, void *hout, void *xh, int S, void *stream);
 int cpr_fno_cast_bf16(const float *x, void *y, int64_t n, void *stream);
 /* y [Cc][R] bf16 = x [R][Cc] f32 transposed. */
 int cpr_fno_cast_bf16_t(const float *x, void *y, int R, int Cc, void *stream);

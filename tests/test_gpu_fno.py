@@ -297,11 +297,14 @@ def test_fused_inference_matches_float64_reference(require_gpu, S):
 
 def test_structured_lift_matches_encoded_input(require_gpu):
     """cpr_fno_lift (the first layer from the encoding's per-query, per-range
-    and per-depth tables) against its float32 formula, against the first
-    layer on the encoded input, and the fused inference through it against
+    and per-depth tables) against its float32 formula and against the first
+    layer on the encoded input; cpr_fno_lift_modes (that layer's modes in
+    closed form, without its output) against the float64 DFT of the formula;
+    and the inference in the table form (forward_fused_lift: modes in closed
+    form, the second layer's pointwise product from the tables) and
     forward_fused on the encoded input, both against the float64 network on
-    the unrounded encoding; the two forms differ by the bf16 rounding of the
-    encoded columns, which the structured form skips."""
+    the unrounded encoding; the forms differ by the bf16 rounding of the
+    encoded columns and of the first layer's output."""
     import importlib
     E = importlib.import_module("cupyram_amd.surrogate.encode")
     from cupyram_amd.surrogate import dataset as D
@@ -328,6 +331,18 @@ def test_structured_lift_matches_encoded_input(require_gpu):
            + zb[:, :, None, None] * lu + below[..., None] * lv).reshape(S * Z, Cw)
     err = (h.float() - ref).abs()
     assert bool((err <= ref.abs() * 2.0 ** -8 + 1e-6).all()), float(err.max())
+    # the first layer's modes without its output (cpr_fno_lift_modes) against
+    # the float64 truncated DFT of the table form
+    tsum, bz = fno._lift_second_tables()[:2]
+    xm = torch.empty_like(xh)
+    k.fno_lift_modes(la, lr, inp.zb, inp.zv, lu, lv, tsum, bz, nr, xm, S)
+    h64 = ((la.double()[:, None, None, :] + lr.double()[None, :, None, :]) + lzt.double().t()[None, None, :, :]
+           + zb.double()[:, :, None, None] * lu.double() + below.double()[..., None] * lv.double()).reshape(S, Z, Cw)
+    an = torch.cat([fno.ac, fno.as_], 0).double()                  # [2K][Z]
+    m64 = torch.einsum("kz,szc->ksc", an, h64)                       # [2K][S][C]
+    xm_ref = torch.cat([m64[:Km], m64[Km:]], dim=2)                  # [K][S][2C]: real, then imaginary parts
+    e_modes = (xm.double() - xm_ref).abs()
+    assert bool((e_modes <= xm_ref.abs() * 2.0 ** -8 + 1e-4 * float(xm_ref.abs().max())).all()), float(e_modes.max())
     h2 = torch.empty_like(h)
     xh2 = torch.empty_like(xh)
     k.fno_layer(x, fno._transposed()["W0"], fno.params["b0"], h2, S, ban=fno.ban, xh=xh2, act=False)
@@ -350,8 +365,11 @@ def test_structured_lift_matches_encoded_input(require_gpu):
     xe[..., E.F_BELOW] = below.double().cpu()
     ref64 = R.forward(R.reference_params(fno), xe.reshape(S, Z, cfg.features), cfg)
     e_lift, e_enc = _rel(p_lift.cpu(), ref64), _rel(p_enc.cpu(), ref64)
-    print(f"structured vs encoded input: first layer {e_h:.2e}, its modes {e_xh:.2e}; prediction vs the float64 "
-          f"network on the unrounded encoding: structured {e_lift:.2e}, encoded input {e_enc:.2e}")
+    e_xm = _rel(xm.float(), xh.float())
+    print(f"structured vs encoded input: first layer {e_h:.2e}, its modes {e_xh:.2e} (closed form vs the kernel's "
+          f"analysis of its bf16 output {e_xm:.2e}); prediction vs the float64 network on the unrounded encoding: "
+          f"table form {e_lift:.2e}, encoded input {e_enc:.2e}")
+    assert e_xm < 1e-2
     assert e_h < 1e-2 and e_xh < 1e-2
     # both within the fused path's bf16 arithmetic of later layers (observed
     # 2.7e-3 and 2.5e-3): the input rounding is not what limits either
