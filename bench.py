@@ -421,8 +421,10 @@ def bench_fno(args):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / n
         mult = 3 if name == "train_step" else 1
-        # the structured first layer does no product with the encoded input
-        mfn = mf - (2 * S * cfg.depth_points * cfg.features * cfg.width if name == "inference_fused_lift" else 0)
+        # the table form does neither the product with the encoded input nor
+        # the second layer's pointwise product (both from the first layer's tables)
+        R = S * cfg.depth_points
+        mfn = mf - (2 * R * cfg.width * (cfg.features + cfg.width) if name == "inference_fused_lift" else 0)
         out[name] = {"ms": dt * 1e3, "fields_per_s": nq / dt, "mfma_tflops": mult * mfn / dt / 1e12,
                      "valu_tflops": mult * vf / dt / 1e12}
     line = {"metric": "surrogate fields/s (200 x 64 TL field per query)",
