@@ -886,7 +886,6 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
     C.T = T;
     C.t = t;
     C.lc = lc;
-    if constexpr (KC > 0) C.fill();
   }
   const MedCache<M> mc{static_cast<double *>(a.mbuf) + static_cast<int64_t>(b) * (M + 2) * 10 * T + t, T};
   // the field in the team layout [ray][k][lane] (rows 1 + t M + k): a team's
@@ -910,6 +909,10 @@ __device__ void march_team_body(const MarchArgs &a, int b, int t, int T, Slot<R>
   bool full = st.dirty != 0;
   if constexpr (!DS) {
     if (!full && s0 < end) load_coefs<R, M>(C, cre, cim, T, t);
+  }
+  if constexpr (DS && KC > 0) {
+    // LDS coefficient rows from the team buffer (a full rebuild writes them)
+    if (!full && s0 < end) C.fill();
   }
   for (int step = s0; step < end; ++step) {
     const double r = (step + 1) * P.dr;
