@@ -584,18 +584,20 @@ struct Coefs<R, M, true, KC> {
 // Rows per lane whose real coefficient halves a team keeps in LDS (streamed
 // modes), sized so that the blocks the VGPRs allow per CU still fit its
 // 160 KB at the lane counts the tuning table gives each row count (complex64
-// 21 x 256 lanes: 19 rows, 78 KB per block; 16 x 128: all 16 rows, 38 KB),
-// and the LDS bytes per lane they take (an odd number of values of R, so
-// that the lanes of a wavefront spread over the banks and every value is
-// aligned to its size).
+// 21 x 256 lanes: 19 rows, 78 KB per block; 16 x 128: all 16 rows, 38 KB;
+// complex128 64-lane teams: all rows up to 12, 296 B per lane), and the LDS
+// bytes per lane KC rows take (an odd number of values of R, so that the
+// lanes of a wavefront spread over the banks and every value is aligned to
+// its size).  complex128 LDS teams of 8 and 10 rows choose their row count
+// at launch (launch_rows).
 template <class R, int M>
 __host__ __device__ constexpr int coef_cache_rows() {
   if (sizeof(R) == 4) return M <= 16 ? M : (M == 21 ? 19 : (M <= 24 ? 17 : 0));
-  return 0;
+  return M <= 12 ? M : 0;
 }
-template <class R, int M>
-__host__ __device__ constexpr int coef_cache_lane_bytes() {
-  const int n = coef_cache_rows<R, M>() * 3;  // values of R
+template <class R>
+__host__ __device__ constexpr int coef_cache_lane_bytes(int kc) {
+  const int n = kc * 3;  // values of R
   return n == 0 ? 0 : (n | 1) * static_cast<int>(sizeof(R));
 }
 
@@ -1214,7 +1216,7 @@ __global__ void __launch_bounds__(256, 2) k_march_team_w(MarchArgs a, int s0, in
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
   Slot<R> *const pb =
       sizeof(R) == 4 ? reinterpret_cast<Slot<R> *>(smem) + (threadIdx.x / 64) * (64 + 2 * kPad) + kPad : nullptr;
-  R *const lc = KC > 0 ? reinterpret_cast<R *>(smem + wave_team_lds_bytes<R>() + threadIdx.x * coef_cache_lane_bytes<R, M>())
+  R *const lc = KC > 0 ? reinterpret_cast<R *>(smem + wave_team_lds_bytes<R>() + threadIdx.x * coef_cache_lane_bytes<R>(KC))
                        : nullptr;
   march_team_body<R, M, 64, DS, M, KC>(a, b, threadIdx.x % 64, 64, pb, lc, s0, s1);
 }
@@ -1231,7 +1233,7 @@ __global__ void __launch_bounds__(512) k_march_team_l(MarchArgs a, int s0, int s
   if (team >= a.team_nrays) return;
   const int b = __builtin_amdgcn_readfirstlane(a.team_rays[team]);
   R *const lc = KC > 0 ? reinterpret_cast<R *>(smem + (team_lds_bytes<R>(blockDim.x) + 15) / 16 * 16 +
-                                                 threadIdx.x * coef_cache_lane_bytes<R, M>())
+                                                 threadIdx.x * coef_cache_lane_bytes<R>(KC))
                        : nullptr;
   march_team_body<R, M, 0, DS, NR, KC>(a, b, threadIdx.x, blockDim.x, reinterpret_cast<Slot<R> *>(smem), lc, s0,
                                        s1);
@@ -1241,15 +1243,25 @@ namespace {
 
 constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS of one CU (gfx950)
 
+// An LDS-mode team with the real coefficient halves of rows 0 .. KC-1 in LDS,
+// launched only if the block's LDS leaves the CU the blocks the VGPRs allow.
+template <class R, int M, bool DS, int KC>
+bool launch_cached(const MarchArgs &a, int lanes, size_t base, size_t blocks, int s0, int s1, hipStream_t st) {
+  const size_t bytes = (base + 15) / 16 * 16 + static_cast<size_t>(lanes) * coef_cache_lane_bytes<R>(KC);
+  if (bytes * blocks > kMaxLdsBytes) return false;
+  hipLaunchKernelGGL((k_march_team_l<R, M, DS, M, KC>), dim3(a.team_nrays), dim3(lanes), bytes, st, a, s0, s1);
+  return true;
+}
+
 template <class R, int M, bool DS>
 int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
   constexpr int KC = DS ? coef_cache_rows<R, M>() : 0;
   if (lanes == 64) {
     const int grid = (a.team_nrays + 3) / 4;
-    const size_t lds = wave_team_lds_bytes<R>();
+    constexpr size_t lds = wave_team_lds_bytes<R>();
     // two blocks per CU (the VGPRs allow two waves per SIMD)
-    const size_t with_cache = lds + 256 * static_cast<size_t>(coef_cache_lane_bytes<R, M>());
-    if (KC > 0 && 2 * with_cache <= kMaxLdsBytes)
+    constexpr size_t with_cache = lds + 256 * static_cast<size_t>(coef_cache_lane_bytes<R>(KC));
+    if constexpr (KC > 0 && 2 * with_cache <= kMaxLdsBytes)
       hipLaunchKernelGGL((k_march_team_w<R, M, DS, KC>), dim3(grid), dim3(256), with_cache, st, a, s0, s1);
     else
       hipLaunchKernelGGL((k_march_team_w<R, M, DS, 0>), dim3(grid), dim3(256), lds, st, a, s0, s1);
@@ -1270,14 +1282,17 @@ int launch_rows(const MarchArgs &a, int lanes, int s0, int s1, hipStream_t st) {
         return hipGetLastError() == hipSuccess ? 0 : -1;
       }
     }
-    if constexpr (KC > 0) {
-      const size_t with_cache =
-          (base + 15) / 16 * 16 + static_cast<size_t>(lanes) * coef_cache_lane_bytes<R, M>();
-      if (with_cache * blocks <= kMaxLdsBytes) {
-        hipLaunchKernelGGL((k_march_team_l<R, M, DS, M, KC>), dim3(a.team_nrays), dim3(lanes), with_cache, st, a, s0,
-                           s1);
+    if constexpr (sizeof(R) == 8 && DS && NR == M && M <= 10) {
+      // complex128 8 and 10 rows: 7, 6 or 4 rows, the most that fit (512,
+      // 256 and 128 lanes); 10 x 256 lanes ran 29.2 instead of 27.6 G
+      // cell-steps/s, 8 x 128 26.2 instead of 24.3 G; 11 x 512 ran 24.1
+      // instead of 25.2 G and keeps its coefficients in the team buffer
+      if (launch_cached<R, M, DS, (M < 7 ? M : 7)>(a, lanes, base, blocks, s0, s1, st) ||
+          launch_cached<R, M, DS, (M < 6 ? M : 6)>(a, lanes, base, blocks, s0, s1, st) ||
+          launch_cached<R, M, DS, (M < 4 ? M : 4)>(a, lanes, base, blocks, s0, s1, st))
         return hipGetLastError() == hipSuccess ? 0 : -1;
-      }
+    } else if constexpr (sizeof(R) == 4 && KC > 0) {
+      if (launch_cached<R, M, DS, KC>(a, lanes, base, blocks, s0, s1, st)) return hipGetLastError() == hipSuccess ? 0 : -1;
     }
     if (base > kMaxLdsBytes) return -1;
     hipLaunchKernelGGL((k_march_team_l<R, M, DS, M, 0>), dim3(a.team_nrays), dim3(lanes), base, st, a, s0, s1);

@@ -219,6 +219,20 @@ def test_lds_tail_rows_match_oracle(require_gpu, env, precision, shape):
         assert worst < 1e-4
 
 
+@pytest.mark.parametrize("env,shape", [("wedge", (8, 128)), ("wedge", (8, 192)), ("wedge", (10, 256)),
+                                       ("munk", (8, 512)), ("munk", (10, 384)), ("munk", (10, 512))])
+def test_c128_lds_coefficient_rows_match_oracle(require_gpu, env, shape):
+    """complex128 LDS teams of 8 and 10 rows per lane keep the real
+    coefficient halves of their first rows in LDS: 4 rows at 128 lanes, 6 at
+    256, 7 at 192, 384 and 512 (what the block's share of the CU's LDS leaves
+    beside the team's buffers); the other rows still come from the team
+    buffer.  Wedge (padding lanes) and the 3,535-row Munk column."""
+    kw = envs.munk_batching(rmax=5000.0) if env == "munk" else envs.wedge(rmax=4000.0)
+    worst = _compare_with_oracle(kw, "c128", "team", 0.1, grids=env == "wedge", team_shape=shape)
+    print(env, shape, "worst rms", worst)
+    assert worst < 1e-4
+
+
 def _uneven(n=6, rmax=3000.0, seed=11):
     from cupyram_amd import workloads
     kw = workloads.uneven_batch(nrays=n, seed=seed, rmax=rmax)
