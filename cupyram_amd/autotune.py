@@ -64,11 +64,13 @@ def tuned_shape(precision: int, nz: int):
     lanes = lanes_for(nz, rows)
     if lanes > 512:
         return None
-    if precision == 64 and lanes == 448:
+    if lanes == 448 and (precision == 64 or rows <= 14):
         # seven waves load the four SIMDs of a CU unevenly: the sweep's
         # complex64 rates at 20, 24, 28 and 40 rows per lane are 19-30 %
         # higher with 512 lanes than with 448, more than the 14 % of rows 512
-        # lanes add (complex128 at 32 rows: 4 %, so it keeps 448)
+        # lanes add; complex128 3-6 % higher at 8, 10 and 14 rows, while its
+        # deeper lanes keep 448 (20 and 24 rows: within 2 %; 32 rows: 6.0
+        # against 5.0 G cell-steps/s)
         lanes = 512
     return rows, lanes
 

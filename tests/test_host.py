@@ -337,6 +337,16 @@ def test_tuning_table_and_static_shape_model(precision):
     assert _abi.team_shape(precision, deepest + 1) == (0, 0)
 
 
+def test_seven_wave_teams_take_512_lanes():
+    """Columns whose tuned row count would give 448-lane teams (seven waves
+    on a CU's four SIMDs) take 512 lanes: every complex64 row count, and
+    complex128 up to 14 rows; deeper complex128 lanes keep 448."""
+    from cupyram_amd import autotune
+    assert autotune.lanes_for(3584, 8) == 448 and autotune.tuned_shape(128, 3584) == (8, 512)
+    assert autotune.lanes_for(8956, 20) == 448 and autotune.tuned_shape(128, 8956) == (20, 448)
+    assert autotune.tuned_shape(64, 8956) == (20, 512)
+
+
 def test_tuning_table_merge_replaces_only_swept_depths():
     """A partial sweep (autotune --nz ... --merge) replaces the entries of its
     precision and depths and keeps the others, sorted by depth."""
